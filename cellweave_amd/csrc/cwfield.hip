@@ -1,0 +1,975 @@
+// cwfield: MI355X (gfx950) diffusion field solve behind a C ABI.
+//
+// Host side of the library: field handles (device mirrors of
+// crates/engine/src/pde.rs:30-53), launch plans, the sweep loop of
+// ScalarField::relax (pde.rs:245-288), FP32 smoothing with FP64 defect
+// correction, and the batched solve. Kernels live in cw_kernels.hpp.
+#include "cw_kernels.hpp"
+#include "../../include/cwfield.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace cw;
+
+namespace {
+
+thread_local std::string g_err;
+
+cw_status fail(cw_status code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define CW_HIP(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return fail(CW_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                             \
+    } while (0)
+
+std::once_flag g_init_once;
+cw_status g_init_status = CW_ERR_DEVICE;
+std::string g_device_name;
+std::string g_version;
+
+cw_status device_init() {
+    std::call_once(g_init_once, [] {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+            g_init_status = CW_ERR_DEVICE;
+            return;
+        }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, 0) != hipSuccess) {
+            g_init_status = CW_ERR_DEVICE;
+            return;
+        }
+        g_device_name = prop.gcnArchName;
+        g_init_status = CW_OK;
+    });
+    if (g_init_status != CW_OK) return fail(CW_ERR_DEVICE, "no usable HIP device (gfx950 expected)");
+    return CW_OK;
+}
+
+constexpr double kPi = 3.14159265358979323846;
+
+// Sweep-tile shapes (interior TX x TY of a 128 x RY region, halo 2s).
+struct TileShape { int tx, ty; };
+TileShape sweep_shape(bool fp64, int s) {
+    if (fp64) return s == 1 ? TileShape{124, 28} : TileShape{120, 24};
+    if (s == 1) return {124, 28};
+    if (s == 2) return {120, 24};
+    return {112, 48};
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Field handle.
+// ---------------------------------------------------------------------------
+struct cw_field {
+    uint32_t W = 0, H = 0, D = 0;
+    int P = 0;
+    size_t n = 0, npad = 0;
+    double diff = 0, decay = 0, source = 0;
+    hipStream_t stream = nullptr;
+    double* c[2] = {nullptr, nullptr};
+    int cur = 0;
+    double* k = nullptr;
+    double* k2 = nullptr;   // uptake, general form only
+    bool general = false;
+    float* e[2] = {nullptr, nullptr};
+    float* r32 = nullptr;
+    float* lb = nullptr;
+    std::vector<uint8_t> held;
+    std::vector<double> uptake, production;
+    bool has_span = false;
+    int span[6] = {0, 0, 0, 0, 0, 0};
+    bool has_sink = false;
+    bool coef_dirty = true;
+};
+
+namespace {
+
+cw_status alloc_zero(void** p, size_t bytes, hipStream_t s, int fill) {
+    CW_HIP(hipMalloc(p, bytes));
+    CW_HIP(hipMemsetAsync(*p, fill, bytes, s));
+    return CW_OK;
+}
+
+void compute_span(cw_field* f) {
+    const size_t w = f->W, h = f->H;
+    bool found = false;
+    for (size_t i = 0; i < f->n; i++) {
+        if (f->held[i]) continue;
+        const int x = (int)(i % w), y = (int)((i / w) % h), z = (int)(i / (w * h));
+        if (!found) {
+            f->span[0] = f->span[1] = x;
+            f->span[2] = f->span[3] = y;
+            f->span[4] = f->span[5] = z;
+            found = true;
+        }
+        f->span[0] = std::min(f->span[0], x);
+        f->span[1] = std::max(f->span[1], x);
+        f->span[2] = std::min(f->span[2], y);
+        f->span[3] = std::max(f->span[3], y);
+        f->span[5] = z;
+    }
+    f->has_span = found;
+}
+
+// Coefficient arrays from held/uptake/production. The folded form k = p - u
+// (p when u == 0, -u when p == 0) reproduces the reference's
+// ((d*sum + source) + p) - u bit for bit when source == 0 and no voxel has
+// both; otherwise k = p and k2 = u are kept apart (general form).
+cw_status build_coefficients(cw_field* f) {
+    bool general = f->source != 0.0;
+    bool sink = false;
+    for (size_t i = 0; i < f->n && !general; i++)
+        if (!f->held[i] && f->production[i] != 0.0 && f->uptake[i] != 0.0) general = true;
+    const double qnan = std::nan("");
+    std::vector<double> k(f->n), k2;
+    if (general) k2.resize(f->n);
+    for (size_t i = 0; i < f->n; i++) {
+        if (f->held[i]) {
+            k[i] = qnan;
+            if (general) k2[i] = 0.0;
+            continue;
+        }
+        const double p = f->production[i], u = f->uptake[i];
+        if (general) {
+            k[i] = p;
+            k2[i] = u;
+            if ((f->source + p) - u < 0.0) sink = true;
+        } else {
+            k[i] = (p != 0.0) ? p : -u;
+            if (k[i] < 0.0) sink = true;
+        }
+    }
+    const size_t pitch = (size_t)f->P * sizeof(double);
+    if (general && !f->k2) {
+        cw_status st = alloc_zero((void**)&f->k2, f->npad * sizeof(double), f->stream, 0);
+        if (st) return st;
+    }
+    CW_HIP(hipMemsetAsync(f->k, 0xFF, f->npad * sizeof(double), f->stream));
+    CW_HIP(hipMemcpy2DAsync(f->k, pitch, k.data(), f->W * sizeof(double), f->W * sizeof(double),
+                            (size_t)f->H * f->D, hipMemcpyHostToDevice, f->stream));
+    if (general)
+        CW_HIP(hipMemcpy2DAsync(f->k2, pitch, k2.data(), f->W * sizeof(double),
+                                f->W * sizeof(double), (size_t)f->H * f->D,
+                                hipMemcpyHostToDevice, f->stream));
+    if (f->r32) CW_HIP(hipMemsetAsync(f->r32, 0xFF, f->npad * sizeof(float), f->stream));
+    CW_HIP(hipStreamSynchronize(f->stream));
+    f->general = general;
+    f->has_sink = sink;
+    f->coef_dirty = false;
+    return CW_OK;
+}
+
+cw_status ensure_fp32(cw_field* f) {
+    if (f->r32) return CW_OK;
+    cw_status st;
+    if ((st = alloc_zero((void**)&f->e[0], f->npad * sizeof(float), f->stream, 0))) return st;
+    if ((st = alloc_zero((void**)&f->e[1], f->npad * sizeof(float), f->stream, 0))) return st;
+    if ((st = alloc_zero((void**)&f->lb, f->npad * sizeof(float), f->stream, 0))) return st;
+    if ((st = alloc_zero((void**)&f->r32, f->npad * sizeof(float), f->stream, 0xFF))) return st;
+    CW_HIP(hipStreamSynchronize(f->stream));
+    return CW_OK;
+}
+
+double reference_omega(const cw_field* f) {
+    if (f->D > 1) {
+        int longest = 0;
+        for (int a = 0; a < 3; a++) longest = std::max(longest, f->span[2 * a + 1] - f->span[2 * a] + 1);
+        longest = std::max(longest, 2);
+        return 2.0 / (1.0 + std::sin(kPi / (double)longest));
+    }
+    const double n = (double)std::max(f->W, f->H);
+    return std::min(2.0 / (1.0 + std::sin(kPi / n)), 1.95);
+}
+
+// ---------------------------------------------------------------------------
+// Launch plan: problems, control blocks, tile tables, reduction scratch.
+// ---------------------------------------------------------------------------
+enum Tiling { TL_MAIN = 0, TL_ONE = 1, TL_EL = 2 };
+
+struct Plan {
+    hipStream_t stream = nullptr;
+    int nprob = 0;
+    int s_main = 1;
+    bool fp64 = true;
+    bool three_d = false;
+    std::vector<Prob> hp;
+    std::vector<Ctl> hc;
+    std::vector<int> tp[3];
+    Prob* dp = nullptr;
+    Ctl* dc = nullptr;
+    int* dtp[3] = {nullptr, nullptr, nullptr};
+    double* partials = nullptr;
+    unsigned* counter = nullptr;
+    size_t cap_prob = 0, cap_tp[3] = {0, 0, 0}, cap_part = 0;
+    // per problem, per tiling: base/count/ntx/nty
+    std::vector<int> base[3], count[3], ntx[3], nty[3];
+
+    ~Plan() { release(); }
+    void release() {
+        hipFree(dp); hipFree(dc); hipFree(partials); hipFree(counter);
+        for (int i = 0; i < 3; i++) hipFree(dtp[i]);
+        dp = nullptr; dc = nullptr; partials = nullptr; counter = nullptr;
+        for (int i = 0; i < 3; i++) dtp[i] = nullptr;
+        cap_prob = cap_part = 0;
+        for (int i = 0; i < 3; i++) cap_tp[i] = 0;
+    }
+};
+
+template <typename T>
+cw_status grow(T** p, size_t* cap, size_t need) {
+    if (need <= *cap) return CW_OK;
+    hipFree(*p);
+    *p = nullptr;
+    size_t c = std::max<size_t>(need, 2 * *cap);
+    CW_HIP(hipMalloc((void**)p, c * sizeof(T)));
+    *cap = c;
+    return CW_OK;
+}
+
+// Lays out the tilings of every problem (main sweep tile, one-sweep tile,
+// elementwise tile) and uploads problems, control blocks and tables.
+cw_status plan_upload(Plan& pl) {
+    const int np = pl.nprob;
+    for (int tl = 0; tl < 3; tl++) {
+        pl.base[tl].assign(np, 0); pl.count[tl].assign(np, 0);
+        pl.ntx[tl].assign(np, 0); pl.nty[tl].assign(np, 0);
+        pl.tp[tl].clear();
+    }
+    for (int p = 0; p < np; p++) {
+        Prob& pr = pl.hp[p];
+        const int sx = pr.x1 - pr.xs + 1, sy = pr.y1 - pr.ys + 1, sz = pr.z1 - pr.zs + 1;
+        for (int tl = 0; tl < 3; tl++) {
+            int nx, ny, nz;
+            if (tl == TL_EL) {
+                nx = (sx + 127) / 128; ny = (sy + 3) / 4; nz = sz;
+            } else {
+                const TileShape ts = sweep_shape(pl.fp64, tl == TL_MAIN ? pl.s_main : 1);
+                nx = (sx + ts.tx - 1) / ts.tx; ny = (sy + ts.ty - 1) / ts.ty; nz = 1;
+                if (pl.three_d) { nx = ny = nz = 0; }
+            }
+            pl.base[tl][p] = (int)pl.tp[tl].size();
+            pl.count[tl][p] = nx * ny * nz;
+            pl.ntx[tl][p] = nx; pl.nty[tl][p] = ny;
+            pl.tp[tl].insert(pl.tp[tl].end(), (size_t)nx * ny * nz, p);
+        }
+    }
+    cw_status st;
+    size_t cap_ctl = pl.cap_prob;
+    if ((st = grow(&pl.dp, &pl.cap_prob, (size_t)std::max(1, np)))) return st;
+    if ((st = grow(&pl.dc, &cap_ctl, (size_t)std::max(1, np)))) return st;
+    size_t maxt = 1;
+    for (int tl = 0; tl < 3; tl++) {
+        maxt = std::max(maxt, pl.tp[tl].size());
+        if (pl.tp[tl].empty()) continue;
+        if ((st = grow(&pl.dtp[tl], &pl.cap_tp[tl], pl.tp[tl].size()))) return st;
+        CW_HIP(hipMemcpyAsync(pl.dtp[tl], pl.tp[tl].data(), pl.tp[tl].size() * sizeof(int),
+                              hipMemcpyHostToDevice, pl.stream));
+    }
+    if ((st = grow(&pl.partials, &pl.cap_part, 2 * maxt))) return st;
+    if (!pl.counter) {
+        CW_HIP(hipMalloc((void**)&pl.counter, sizeof(unsigned)));
+    }
+    CW_HIP(hipMemsetAsync(pl.counter, 0, sizeof(unsigned), pl.stream));
+    CW_HIP(hipMemcpyAsync(pl.dc, pl.hc.data(), np * sizeof(Ctl), hipMemcpyHostToDevice, pl.stream));
+    return CW_OK;
+}
+
+// Problem descriptors carry one tiling at a time (the kernels read
+// sw_base/ntx for sweeps, el_base/elx for elementwise work).
+cw_status plan_set_tiling(Plan& pl, int tl) {
+    for (int p = 0; p < pl.nprob; p++) {
+        Prob& pr = pl.hp[p];
+        if (tl != TL_EL) {
+            pr.sw_base = pl.base[tl][p]; pr.sw_count = pl.count[tl][p];
+            pr.ntx = pl.ntx[tl][p]; pr.nty = pl.nty[tl][p]; pr.ntz = 1;
+        }
+        pr.el_base = pl.base[TL_EL][p]; pr.el_count = pl.count[TL_EL][p];
+        pr.elx = pl.ntx[TL_EL][p]; pr.ely = pl.nty[TL_EL][p];
+    }
+    CW_HIP(hipMemcpyAsync(pl.dp, pl.hp.data(), pl.nprob * sizeof(Prob), hipMemcpyHostToDevice,
+                          pl.stream));
+    return CW_OK;
+}
+
+PassArgs make_args(Plan& pl, int tl, int mode, int crit, int check, int s, double tol,
+                   int max_sweeps) {
+    PassArgs a{};
+    a.probs = pl.dp;
+    a.ctl = pl.dc;
+    a.tile_prob = pl.dtp[tl];
+    a.partials = pl.partials;
+    a.counter = pl.counter;
+    a.nprob = pl.nprob;
+    a.ntiles = (int)pl.tp[tl].size();
+    a.crit = crit;
+    a.check = check;
+    a.mode = mode;
+    a.max_sweeps = max_sweeps;
+    a.s = s;
+    a.tiling = (tl == TL_EL) ? 1 : 0;
+    a.tol = tol;
+    a.inner_ctl = nullptr;
+    return a;
+}
+
+cw_status launch_sweep(Plan& pl, int tl, int s, bool plain, const PassArgs& a) {
+    const int nt = (int)pl.tp[tl].size();
+    if (nt == 0) return CW_OK;
+    dim3 grid(nt);
+    if (pl.fp64) {
+        if (s == 1) {
+            if (plain) hipLaunchKernelGGL((sweep2d<double, 1, 32, 4, true>), grid, dim3(256), 0, pl.stream, a);
+            else hipLaunchKernelGGL((sweep2d<double, 1, 32, 4, false>), grid, dim3(256), 0, pl.stream, a);
+        } else {
+            if (plain) hipLaunchKernelGGL((sweep2d<double, 2, 32, 4, true>), grid, dim3(256), 0, pl.stream, a);
+            else hipLaunchKernelGGL((sweep2d<double, 2, 32, 4, false>), grid, dim3(256), 0, pl.stream, a);
+        }
+    } else {
+        if (s == 1) hipLaunchKernelGGL((sweep2d<float, 1, 32, 4, false>), grid, dim3(256), 0, pl.stream, a);
+        else if (s == 2) hipLaunchKernelGGL((sweep2d<float, 2, 32, 4, false>), grid, dim3(256), 0, pl.stream, a);
+        else hipLaunchKernelGGL((sweep2d<float, 4, 64, 8, false>), grid, dim3(512), 0, pl.stream, a);
+    }
+    CW_HIP(hipGetLastError());
+    return CW_OK;
+}
+
+cw_status launch_half3d(Plan& pl, bool plain, PassArgs a, int colour) {
+    const int nt = (int)pl.tp[TL_EL].size();
+    if (nt == 0) return CW_OK;
+    a.tile_prob = pl.dtp[TL_EL];
+    a.tiling = 1;
+    if (pl.fp64) {
+        if (plain) hipLaunchKernelGGL((half3d<double, true>), dim3(nt), dim3(256), 0, pl.stream, a, colour);
+        else hipLaunchKernelGGL((half3d<double, false>), dim3(nt), dim3(256), 0, pl.stream, a, colour);
+    } else {
+        hipLaunchKernelGGL((half3d<float, false>), dim3(nt), dim3(256), 0, pl.stream, a, colour);
+    }
+    CW_HIP(hipGetLastError());
+    return CW_OK;
+}
+
+cw_status read_ctl(Plan& pl) {
+    CW_HIP(hipMemcpyAsync(pl.hc.data(), pl.dc, pl.nprob * sizeof(Ctl), hipMemcpyDeviceToHost,
+                          pl.stream));
+    CW_HIP(hipStreamSynchronize(pl.stream));
+    return CW_OK;
+}
+
+cw_status write_ctl(Plan& pl) {
+    CW_HIP(hipMemcpyAsync(pl.dc, pl.hc.data(), pl.nprob * sizeof(Ctl), hipMemcpyHostToDevice,
+                          pl.stream));
+    return CW_OK;
+}
+
+bool any_status(const Plan& pl, int st) {
+    for (int p = 0; p < pl.nprob; p++)
+        if (pl.hc[p].status == st) return true;
+    return false;
+}
+
+struct LoopResult { long passes = 0; };
+
+// Runs passes of the plan until no problem is RUNNING or `max_sweeps` sweeps
+// have run in this call; the device marks a problem MAXED once its own count
+// reaches dev_max. Rolls back and replays exact stops inside a pass.
+cw_status run_passes(Plan& pl, bool plain, int crit, double tol, uint64_t max_sweeps,
+                     uint64_t dev_max, int check_every, LoopResult* out) {
+    const int s = pl.s_main;
+    uint64_t host_sweeps = 0;
+    long pass_idx = 0;
+    int chunk = 4;
+    cw_status st;
+    const int maxs = (int)std::min<uint64_t>(dev_max, 0x7fffffff);
+    while (true) {
+        for (int j = 0; j < chunk; j++) {
+            if (host_sweeps >= max_sweeps) break;
+            const uint64_t left = max_sweeps - host_sweeps;
+            const int check = (crit == CRIT_RESID || crit == CRIT_INNER)
+                                  ? (pass_idx % check_every == 0)
+                                  : 0;
+            if (pl.three_d) {
+                PassArgs a = make_args(pl, TL_EL, MODE_HALF_RED, crit, 0, 1, tol, maxs);
+                if ((st = launch_half3d(pl, plain, a, 0))) return st;
+                a.mode = MODE_HALF_BLACK;
+                if ((st = launch_half3d(pl, plain, a, 1))) return st;
+                host_sweeps += 1;
+            } else if ((uint64_t)s <= left) {
+                PassArgs a = make_args(pl, TL_MAIN, MODE_SWEEP, crit, check, s, tol, maxs);
+                if ((st = launch_sweep(pl, TL_MAIN, s, plain, a))) return st;
+                host_sweeps += s;
+            } else {
+                if ((st = plan_set_tiling(pl, TL_ONE))) return st;
+                PassArgs a = make_args(pl, TL_ONE, MODE_SWEEP, crit, check, 1, tol, maxs);
+                if ((st = launch_sweep(pl, TL_ONE, 1, plain, a))) return st;
+                if ((st = plan_set_tiling(pl, TL_MAIN))) return st;
+                host_sweeps += 1;
+            }
+            pass_idx++;
+        }
+        if ((st = read_ctl(pl))) return st;
+        if (!any_status(pl, ST_RUNNING) || host_sweeps >= max_sweeps) break;
+        chunk = std::min(chunk * 2, 64);
+    }
+    // Exact stop inside a fused pass: replay single sweeps from the kept input.
+    if (any_status(pl, ST_REPLAY)) {
+        int most = 0;
+        for (int p = 0; p < pl.nprob; p++)
+            if (pl.hc[p].status == ST_REPLAY) most = std::max(most, pl.hc[p].replay);
+        if ((st = plan_set_tiling(pl, TL_ONE))) return st;
+        for (int r = 0; r < most; r++) {
+            PassArgs a = make_args(pl, TL_ONE, MODE_REPLAY, crit, 0, 1, tol, maxs);
+            if ((st = launch_sweep(pl, TL_ONE, 1, plain, a))) return st;
+        }
+        if ((st = plan_set_tiling(pl, TL_MAIN))) return st;
+        if ((st = read_ctl(pl))) return st;
+    }
+    if (out) out->passes = pass_idx;
+    return CW_OK;
+}
+
+cw_status run_resid(Plan& pl, int crit, double tol, int write32) {
+    const int nt = (int)pl.tp[TL_EL].size();
+    if (nt == 0) return CW_OK;
+    PassArgs a = make_args(pl, TL_EL, MODE_RESID, crit, 1, 1, tol, 0);
+    hipLaunchKernelGGL(resid64, dim3(nt), dim3(256), 0, pl.stream, a, write32);
+    CW_HIP(hipGetLastError());
+    return CW_OK;
+}
+
+void fill_prob_common(Prob& pr, const cw_field* f, double omega) {
+    memset(&pr, 0, sizeof pr);
+    pr.W = (int)f->W; pr.H = (int)f->H; pr.D = (int)f->D; pr.P = f->P;
+    pr.xs = f->span[0] & ~1; pr.ys = f->span[2]; pr.zs = f->span[4];
+    pr.x1 = f->span[1]; pr.y1 = f->span[3]; pr.z1 = f->span[5];
+    pr.d = f->diff; pr.decay = f->decay; pr.source = f->source;
+    pr.omega = omega; pr.omo = 1.0 - omega;
+    for (int r = 0; r < 8; r++) pr.inv_den[r] = (float)(1.0 / ((double)r * f->diff + f->decay));
+}
+
+void init_ctl(Ctl& c, int status) {
+    memset(&c, 0, sizeof c);
+    c.status = status;
+}
+
+struct Opts {
+    double omega;
+    int crit;
+    double tol;
+    uint64_t max_sweeps;
+    int precision;
+    int s;
+    int check_every;
+    double inner_tol;
+    uint64_t max_outer;
+};
+
+cw_status check_opts(const cw_relax_opts* o, Opts* out) {
+    if (!o) return fail(CW_ERR_ARG, "null options");
+    if (o->criterion != CW_CRIT_MAX_CHANGE && o->criterion != CW_CRIT_REL_RESIDUAL)
+        return fail(CW_ERR_ARG, "unknown criterion %d", o->criterion);
+    if (o->precision != CW_PREC_FP64 && o->precision != CW_PREC_FP32_REFINE)
+        return fail(CW_ERR_ARG, "unknown precision %d", o->precision);
+    if (o->precision == CW_PREC_FP32_REFINE && o->criterion != CW_CRIT_REL_RESIDUAL)
+        return fail(CW_ERR_ARG, "FP32 refinement stops on the relative residual only");
+    if (!(o->tolerance == o->tolerance)) return fail(CW_ERR_ARG, "tolerance is NaN");
+    if (o->omega > 0.0 && !(o->omega < 2.0))
+        return fail(CW_ERR_ARG, "omega %g outside (0, 2)", o->omega);
+    if (o->sweeps_per_pass < 0 || o->sweeps_per_pass > 4 || o->sweeps_per_pass == 3)
+        return fail(CW_ERR_ARG, "sweeps_per_pass must be 0, 1, 2 or 4");
+    out->omega = o->omega;
+    out->crit = o->criterion == CW_CRIT_MAX_CHANGE ? CRIT_MAXCHANGE : CRIT_RESID;
+    out->tol = o->tolerance;
+    out->max_sweeps = o->max_sweeps;
+    out->precision = o->precision;
+    out->s = o->sweeps_per_pass;
+    out->check_every = o->check_every > 0 ? o->check_every : 0;
+    out->inner_tol = o->inner_tolerance > 0.0 ? o->inner_tolerance : 1e-4;
+    out->max_outer = o->max_outer > 0 ? o->max_outer : 40;
+    return CW_OK;
+}
+
+// Plain FP64 solve of a set of fields (each sweeps its own span box).
+cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_stats* stats,
+                     bool* all_ok) {
+    cw_status st;
+    hipStream_t stream = F[0]->stream;
+    const bool three_d = F[0]->D > 1;
+    for (uint32_t i = 0; i < nf; i++)
+        if ((F[i]->D > 1) != three_d)
+            return fail(CW_ERR_ARG, "a batch mixes planes and volumes");
+    int s = o.s ? o.s : 2;
+    if (s > 2) s = 2;
+    if (three_d) s = 1;
+    Plan pl;
+    pl.stream = stream;
+    pl.fp64 = true;
+    pl.three_d = three_d;
+    pl.s_main = s;
+    std::vector<int> idx;  // plan problem -> field
+    std::vector<double> omegas(nf);
+    for (uint32_t i = 0; i < nf; i++) {
+        cw_field* f = F[i];
+        if (stats) {
+            memset(&stats[i], 0, sizeof(cw_relax_stats));
+            stats[i].rel_residual = std::nan("");
+            stats[i].sweeps_per_pass = s;
+        }
+        if (!f->has_span) {
+            if (stats) { stats[i].sweeps = o.crit == CRIT_MAXCHANGE ? 1 : 0; stats[i].status = CW_OK; }
+            continue;
+        }
+        omegas[i] = o.omega > 0.0 ? o.omega : reference_omega(f);
+        if (o.max_sweeps == 0) {
+            if (stats) stats[i].status = CW_ERR_NOT_CONVERGED;
+            continue;
+        }
+        // Outside the swept box both buffers must agree (pde.rs:249-252, :278).
+        CW_HIP(hipMemcpyAsync(f->c[f->cur ^ 1], f->c[f->cur], f->npad * sizeof(double),
+                              hipMemcpyDeviceToDevice, f->stream));
+        Prob pr;
+        fill_prob_common(pr, f, omegas[i]);
+        pr.x[0] = f->c[f->cur];
+        pr.x[1] = three_d ? f->c[f->cur] : f->c[f->cur ^ 1];
+        pr.in_place = three_d;
+        pr.k = f->k;
+        pr.k2 = f->general ? f->k2 : nullptr;
+        pr.c64 = f->c[f->cur];
+        pl.hp.push_back(pr);
+        Ctl c;
+        init_ctl(c, ST_RUNNING);
+        pl.hc.push_back(c);
+        idx.push_back((int)i);
+    }
+    for (uint32_t i = 0; i < nf; i++)
+        if (F[i]->stream != stream) CW_HIP(hipStreamSynchronize(F[i]->stream));
+    pl.nprob = (int)idx.size();
+    bool ok = true;
+    if (pl.nprob > 0) {
+        if ((st = plan_upload(pl))) return st;
+        if ((st = plan_set_tiling(pl, TL_MAIN))) return st;
+        int check_every = o.check_every ? o.check_every : (three_d ? 8 : 1);
+        LoopResult lr;
+        if (o.crit == CRIT_RESID) {
+            if ((st = run_resid(pl, CRIT_RESID, o.tol, 0))) return st;
+            if ((st = read_ctl(pl))) return st;
+        }
+        if (three_d && o.crit == CRIT_RESID) {
+            // Volumes test the residual in its own pass every check_every sweeps.
+            uint64_t done = 0;
+            while (any_status(pl, ST_RUNNING) && done < o.max_sweeps) {
+                const uint64_t n = std::min<uint64_t>(check_every, o.max_sweeps - done);
+                if ((st = run_passes(pl, true, CRIT_NONE, o.tol, n, done + n, 1, &lr))) return st;
+                // run_passes stops problems at the running max; re-arm the ones maxed early.
+                for (int p = 0; p < pl.nprob; p++)
+                    if (pl.hc[p].status == ST_MAXED && (uint64_t)pl.hc[p].sweeps < o.max_sweeps &&
+                        !pl.hc[p].nonfinite)
+                        pl.hc[p].status = ST_RUNNING;
+                if ((st = write_ctl(pl))) return st;
+                done += n;
+                if ((st = run_resid(pl, CRIT_RESID, o.tol, 0))) return st;
+                if ((st = read_ctl(pl))) return st;
+            }
+        } else {
+            if ((st = run_passes(pl, true, o.crit, o.tol, o.max_sweeps, o.max_sweeps, check_every,
+                                 &lr)))
+                return st;
+        }
+        for (int p = 0; p < pl.nprob; p++) {
+            cw_field* f = F[idx[p]];
+            const Ctl& c = pl.hc[p];
+            if (!three_d && (c.pass & 1)) f->cur ^= 1;
+            cw_status fs = CW_OK;
+            if (c.nonfinite) fs = CW_ERR_NONFINITE;
+            else if (c.status != ST_DONE) fs = CW_ERR_NOT_CONVERGED;
+            if (fs != CW_OK) ok = false;
+            if (stats) {
+                cw_relax_stats& S = stats[idx[p]];
+                S.sweeps = (uint64_t)c.sweeps;
+                S.passes = (uint64_t)lr.passes;
+                S.status = fs;
+                S.omega = omegas[idx[p]];
+                S.span_voxels = (uint64_t)(f->span[1] - f->span[0] + 1) *
+                                (f->span[3] - f->span[2] + 1) * (f->span[5] - f->span[4] + 1);
+                if (o.crit == CRIT_RESID && c.b2 > 0.0) S.rel_residual = std::sqrt(c.r2 / c.b2);
+            }
+        }
+    }
+    for (uint32_t i = 0; i < nf; i++) {
+        if (stats && stats[i].status != CW_OK) ok = false;
+    }
+    *all_ok = ok;
+    return CW_OK;
+}
+
+// FP32 smoothing of the defect equation with FP64 residual and correction.
+//   r = b - A c (FP64), A e = r smoothed in FP32 with e >= -c (projected SOR),
+//   c <- max(c + e, 0) (FP64), until ||rt|| / ||b_eff|| <= tol.
+cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_stats* stats,
+                       bool* all_ok) {
+    cw_status st;
+    hipStream_t stream = F[0]->stream;
+    const bool three_d = F[0]->D > 1;
+    for (uint32_t i = 0; i < nf; i++)
+        if ((F[i]->D > 1) != three_d)
+            return fail(CW_ERR_ARG, "a batch mixes planes and volumes");
+    int s = o.s ? o.s : 2;
+    if (three_d) s = 1;
+    Plan outer, inner;
+    outer.stream = inner.stream = stream;
+    outer.fp64 = true;
+    inner.fp64 = false;
+    outer.three_d = inner.three_d = three_d;
+    outer.s_main = 1;
+    inner.s_main = s;
+    std::vector<int> idx;
+    std::vector<double> omegas(nf);
+    for (uint32_t i = 0; i < nf; i++) {
+        cw_field* f = F[i];
+        if (stats) {
+            memset(&stats[i], 0, sizeof(cw_relax_stats));
+            stats[i].rel_residual = std::nan("");
+            stats[i].sweeps_per_pass = s;
+        }
+        if (!f->has_span) continue;
+        if ((st = ensure_fp32(f))) return st;
+        omegas[i] = o.omega > 0.0 ? o.omega : reference_omega(f);
+        Prob pr;
+        fill_prob_common(pr, f, omegas[i]);
+        pr.k = f->k;
+        pr.k2 = f->general ? f->k2 : nullptr;
+        pr.c64 = f->c[f->cur];
+        pr.e0 = f->e[0];
+        pr.e1 = f->e[1];
+        pr.r32 = f->r32;
+        pr.lb32 = f->has_sink ? f->lb : nullptr;
+        pr.inner_idx = (int)idx.size();
+        outer.hp.push_back(pr);
+        Prob ip = pr;
+        ip.x[0] = f->e[0];
+        ip.x[1] = three_d ? f->e[0] : f->e[1];
+        ip.in_place = three_d;
+        ip.k = f->r32;
+        ip.k2 = nullptr;
+        ip.lb = pr.lb32;
+        inner.hp.push_back(ip);
+        Ctl c;
+        init_ctl(c, ST_RUNNING);
+        outer.hc.push_back(c);
+        init_ctl(c, ST_IDLE);
+        inner.hc.push_back(c);
+        idx.push_back((int)i);
+    }
+    outer.nprob = inner.nprob = (int)idx.size();
+    std::vector<uint64_t> inner_sweeps(nf, 0), outer_iters(nf, 0);
+    long passes = 0;
+    if (outer.nprob > 0) {
+        if ((st = plan_upload(outer))) return st;
+        if ((st = plan_set_tiling(outer, TL_MAIN))) return st;
+        if ((st = plan_upload(inner))) return st;
+        if ((st = plan_set_tiling(inner, TL_MAIN))) return st;
+        const int check_every = o.check_every ? o.check_every : (three_d ? 8 : 2);
+        if ((st = run_resid(outer, CRIT_RESID, o.tol, 1))) return st;
+        if ((st = read_ctl(outer))) return st;
+        uint64_t total_inner = 0;
+        for (uint64_t it = 0; it < o.max_outer && any_status(outer, ST_RUNNING); it++) {
+            const uint64_t left = o.max_sweeps > total_inner ? o.max_sweeps - total_inner : 0;
+            if (left == 0) break;
+            for (int p = 0; p < inner.nprob; p++) {
+                init_ctl(inner.hc[p], outer.hc[p].status == ST_RUNNING ? ST_RUNNING : ST_IDLE);
+                if (outer.hc[p].status == ST_RUNNING) outer_iters[idx[p]]++;
+            }
+            if ((st = write_ctl(inner))) return st;
+            LoopResult lr;
+            // Volumes smooth a fixed count per step, the asymptotic SOR rate
+            // rho = omega - 1 brought down by inner_tol (with 50% margin).
+            uint64_t budget = left;
+            if (three_d) {
+                double rho = std::max(0.5, omegas[idx[0]] - 1.0);
+                budget = std::min<uint64_t>(
+                    left, (uint64_t)(1.5 * std::log(o.inner_tol) / std::log(rho)) + 8);
+            }
+            if ((st = run_passes(inner, false, three_d ? CRIT_NONE : CRIT_INNER, o.inner_tol,
+                                 budget, budget, check_every, &lr)))
+                return st;
+            passes += lr.passes;
+            uint64_t most = 0;
+            for (int p = 0; p < inner.nprob; p++) {
+                if (outer.hc[p].status != ST_RUNNING) continue;
+                inner_sweeps[idx[p]] += (uint64_t)inner.hc[p].sweeps;
+                most = std::max<uint64_t>(most, (uint64_t)inner.hc[p].sweeps);
+            }
+            total_inner += most;
+            {
+                const int nt = (int)outer.tp[TL_EL].size();
+                PassArgs a = make_args(outer, TL_EL, MODE_PLAIN, CRIT_NONE, 0, 1, 0.0, 0);
+                a.inner_ctl = inner.dc;
+                hipLaunchKernelGGL(correct64, dim3(nt), dim3(256), 0, stream, a);
+                CW_HIP(hipGetLastError());
+            }
+            if ((st = run_resid(outer, CRIT_RESID, o.tol, 1))) return st;
+            if ((st = read_ctl(outer))) return st;
+        }
+    }
+    bool ok = true;
+    for (int p = 0; p < outer.nprob; p++) {
+        const Ctl& c = outer.hc[p];
+        cw_status fs = CW_OK;
+        if (c.nonfinite) fs = CW_ERR_NONFINITE;
+        else if (c.status != ST_DONE) fs = CW_ERR_NOT_CONVERGED;
+        if (fs != CW_OK) ok = false;
+        if (stats) {
+            cw_relax_stats& S = stats[idx[p]];
+            cw_field* f = F[idx[p]];
+            S.sweeps = inner_sweeps[idx[p]];
+            S.passes = (uint64_t)passes;
+            S.outer_iters = outer_iters[idx[p]];
+            S.status = fs;
+            S.omega = omegas[idx[p]];
+            S.span_voxels = (uint64_t)(f->span[1] - f->span[0] + 1) *
+                            (f->span[3] - f->span[2] + 1) * (f->span[5] - f->span[4] + 1);
+            if (c.b2 > 0.0) S.rel_residual = std::sqrt(c.r2 / c.b2);
+        }
+    }
+    *all_ok = ok;
+    return CW_OK;
+}
+
+cw_status solve(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts, cw_relax_stats* stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!F || nf == 0) return fail(CW_ERR_ARG, "no fields");
+    Opts o;
+    cw_status st = check_opts(opts, &o);
+    if (st) return st;
+    for (uint32_t i = 0; i < nf; i++) {
+        if (!F[i]) return fail(CW_ERR_ARG, "null field %u", i);
+        if (F[i]->coef_dirty && (st = build_coefficients(F[i]))) return st;
+    }
+    std::vector<cw_relax_stats> local;
+    if (!stats) {
+        local.resize(nf);
+        stats = local.data();
+    }
+    bool ok = true;
+    if (o.precision == CW_PREC_FP64) st = solve_fp64(F, nf, o, stats, &ok);
+    else st = solve_refine(F, nf, o, stats, &ok);
+    if (st) return st;
+    CW_HIP(hipStreamSynchronize(F[0]->stream));
+    const double secs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    cw_status worst = CW_OK;
+    for (uint32_t i = 0; i < nf; i++) {
+        stats[i].seconds = secs;
+        if (stats[i].status != CW_OK && worst == CW_OK) worst = (cw_status)stats[i].status;
+    }
+    if (worst == CW_ERR_NOT_CONVERGED) {
+        for (uint32_t i = 0; i < nf; i++)
+            if (stats[i].status == CW_ERR_NOT_CONVERGED)
+                return fail(CW_ERR_NOT_CONVERGED,
+                            "diffusion did not settle within %llu sweeps at tolerance %g",
+                            (unsigned long long)o.max_sweeps, o.tol);
+    }
+    if (worst == CW_ERR_NONFINITE) return fail(CW_ERR_NONFINITE, "the residual is not finite");
+    (void)ok;
+    return CW_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void cw_relax_opts_default(cw_relax_opts* o) {
+    if (!o) return;
+    memset(o, 0, sizeof *o);
+    o->omega = 0.0;
+    o->criterion = CW_CRIT_MAX_CHANGE;
+    o->tolerance = 1e-8;
+    o->max_sweeps = 2000000;
+    o->precision = CW_PREC_FP64;
+    o->sweeps_per_pass = 0;
+    o->check_every = 0;
+    o->inner_tolerance = 0.0;
+    o->max_outer = 0;
+}
+
+const char* cw_last_error(void) { return g_err.c_str(); }
+
+const char* cw_version(void) {
+    if (g_version.empty()) {
+        g_version = "cwfield 0.1.0 gfx950";
+        if (device_init() == CW_OK) g_version += " on " + g_device_name;
+    }
+    return g_version.c_str();
+}
+
+cw_status cw_field_create(uint32_t w, uint32_t h, uint32_t d, double diff, double decay,
+                          double initial, cw_field** out) {
+    if (!out) return fail(CW_ERR_ARG, "null output handle");
+    *out = nullptr;
+    if (w == 0 || h == 0 || d == 0)
+        return fail(CW_ERR_CONFIG, "field dimensions must be positive, got %ux%ux%u", w, h, d);
+    if (!std::isfinite(diff) || !std::isfinite(decay) || diff < 0.0 || decay < 0.0)
+        return fail(CW_ERR_CONFIG,
+                    "diffusion and decay must be finite and not negative, got D = %g, decay = %g",
+                    diff, decay);
+    if ((uint64_t)w * h * d > (1ull << 34)) return fail(CW_ERR_CONFIG, "field too large");
+    cw_status st = device_init();
+    if (st) return st;
+    cw_field* f = new cw_field();
+    f->W = w; f->H = h; f->D = d;
+    f->P = (int)((w + 63) / 64 * 64);
+    f->n = (size_t)w * h * d;
+    f->npad = (size_t)f->P * h * d;
+    f->diff = diff; f->decay = decay;
+    f->held.assign(f->n, 0);
+    f->uptake.assign(f->n, 0.0);
+    f->production.assign(f->n, 0.0);
+    auto bail = [&](cw_status s) { cw_field_destroy(f); return s; };
+    if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(CW_ERR_DEVICE, "hipStreamCreate failed"));
+    if ((st = alloc_zero((void**)&f->c[0], f->npad * sizeof(double), f->stream, 0))) return bail(st);
+    if ((st = alloc_zero((void**)&f->c[1], f->npad * sizeof(double), f->stream, 0))) return bail(st);
+    if ((st = alloc_zero((void**)&f->k, f->npad * sizeof(double), f->stream, 0xFF))) return bail(st);
+    std::vector<double> init(f->n, initial);
+    if (hipMemcpy2DAsync(f->c[0], f->P * sizeof(double), init.data(), w * sizeof(double),
+                         w * sizeof(double), (size_t)h * d, hipMemcpyHostToDevice,
+                         f->stream) != hipSuccess ||
+        hipStreamSynchronize(f->stream) != hipSuccess)
+        return bail(fail(CW_ERR_DEVICE, "initial upload failed"));
+    compute_span(f);
+    f->coef_dirty = true;
+    *out = f;
+    return CW_OK;
+}
+
+cw_status cw_field_destroy(cw_field* f) {
+    if (!f) return CW_OK;
+    if (f->stream) hipStreamSynchronize(f->stream);
+    hipFree(f->c[0]); hipFree(f->c[1]); hipFree(f->k); hipFree(f->k2);
+    hipFree(f->e[0]); hipFree(f->e[1]); hipFree(f->r32); hipFree(f->lb);
+    if (f->stream) hipStreamDestroy(f->stream);
+    delete f;
+    return CW_OK;
+}
+
+cw_status cw_field_set_source(cw_field* f, double source) {
+    if (!f) return fail(CW_ERR_ARG, "null field");
+    f->source = source;
+    f->coef_dirty = true;
+    return CW_OK;
+}
+
+cw_status cw_field_upload(cw_field* f, const double* values, const uint8_t* held,
+                          const double* uptake, const double* production) {
+    if (!f) return fail(CW_ERR_ARG, "null field");
+    if (held) {
+        for (size_t i = 0; i < f->n; i++) f->held[i] = held[i] ? 1 : 0;
+        compute_span(f);
+        f->coef_dirty = true;
+    }
+    // The setters clamp at zero (pde.rs:146-182); NaN clamps to zero as
+    // f64::max(NaN, 0.0) does.
+    if (uptake) {
+        for (size_t i = 0; i < f->n; i++) f->uptake[i] = uptake[i] > 0.0 ? uptake[i] : 0.0;
+        f->coef_dirty = true;
+    }
+    if (production) {
+        for (size_t i = 0; i < f->n; i++) f->production[i] = production[i] > 0.0 ? production[i] : 0.0;
+        f->coef_dirty = true;
+    }
+    if (values) {
+        CW_HIP(hipMemcpy2DAsync(f->c[f->cur], f->P * sizeof(double), values,
+                                f->W * sizeof(double), f->W * sizeof(double),
+                                (size_t)f->H * f->D, hipMemcpyHostToDevice, f->stream));
+        CW_HIP(hipStreamSynchronize(f->stream));
+    }
+    if (f->coef_dirty) return build_coefficients(f);
+    return CW_OK;
+}
+
+cw_status cw_field_download(const cw_field* f, double* values) {
+    if (!f || !values) return fail(CW_ERR_ARG, "null argument");
+    CW_HIP(hipMemcpy2DAsync(values, f->W * sizeof(double), f->c[f->cur], f->P * sizeof(double),
+                            f->W * sizeof(double), (size_t)f->H * f->D, hipMemcpyDeviceToHost,
+                            f->stream));
+    CW_HIP(hipStreamSynchronize(f->stream));
+    return CW_OK;
+}
+
+cw_status cw_field_relax(cw_field* f, double tolerance, uint64_t max_sweeps, uint64_t* sweeps) {
+    if (!f) return fail(CW_ERR_ARG, "null field");
+    cw_relax_opts o;
+    cw_relax_opts_default(&o);
+    o.tolerance = tolerance;
+    o.max_sweeps = max_sweeps;
+    cw_relax_stats s;
+    cw_status st = solve(&f, 1, &o, &s);
+    if (sweeps) *sweeps = s.sweeps;
+    return st;
+}
+
+cw_status cw_field_relax_ex(cw_field* f, const cw_relax_opts* opts, cw_relax_stats* stats) {
+    if (!f) return fail(CW_ERR_ARG, "null field");
+    return solve(&f, 1, opts, stats);
+}
+
+cw_status cw_batch_relax(cw_field* const* fields, uint32_t count, const cw_relax_opts* opts,
+                         cw_relax_stats* stats) {
+    return solve(fields, count, opts, stats);
+}
+
+cw_status cw_field_residual(cw_field* f, double* rel) {
+    if (!f || !rel) return fail(CW_ERR_ARG, "null argument");
+    cw_status st;
+    if (f->coef_dirty && (st = build_coefficients(f))) return st;
+    if (!f->has_span) { *rel = 0.0; return CW_OK; }
+    Plan pl;
+    pl.stream = f->stream;
+    pl.fp64 = true;
+    pl.three_d = f->D > 1;
+    pl.s_main = 1;
+    Prob pr;
+    fill_prob_common(pr, f, 1.0);
+    pr.k = f->k;
+    pr.k2 = f->general ? f->k2 : nullptr;
+    pr.c64 = f->c[f->cur];
+    pl.hp.push_back(pr);
+    Ctl c;
+    init_ctl(c, ST_RUNNING);
+    pl.hc.push_back(c);
+    pl.nprob = 1;
+    if ((st = plan_upload(pl))) return st;
+    if ((st = plan_set_tiling(pl, TL_MAIN))) return st;
+    if ((st = run_resid(pl, CRIT_NONE, 0.0, 0))) return st;
+    if ((st = read_ctl(pl))) return st;
+    const Ctl& r = pl.hc[0];
+    *rel = r.b2 > 0.0 ? std::sqrt(r.r2 / r.b2) : std::sqrt(r.r2);
+    return CW_OK;
+}
+
+}  // extern "C"

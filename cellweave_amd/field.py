@@ -1,0 +1,151 @@
+"""Python handle on the GPU field solve (thin layer over the C ABI).
+
+Field mirrors the device side of cellweave's ScalarField
+(crates/engine/src/pde.rs:30-53): the caller owns the host arrays and uploads
+them in bulk, relax() runs on the GPU, download() brings the values back.
+Errors map onto the reference's two variants: ConfigError for
+CellweaveError::Config and SimulationError for CellweaveError::Simulation
+(crates/core/src/error.rs:7-14).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import (CRIT_MAX_CHANGE, CRIT_REL_RESIDUAL, PREC_FP32_REFINE, PREC_FP64,
+                   RelaxOpts, RelaxStats)
+
+
+class CellweaveError(RuntimeError):
+    pass
+
+
+class ConfigError(CellweaveError):
+    """CellweaveError::Config."""
+
+
+class SimulationError(CellweaveError):
+    """CellweaveError::Simulation: the field did not settle."""
+
+
+class DeviceError(CellweaveError):
+    pass
+
+
+def _check(rc: int):
+    if rc == _abi.CW_OK:
+        return
+    msg = _abi.last_error()
+    if rc == _abi.CW_ERR_CONFIG:
+        raise ConfigError(msg)
+    if rc == _abi.CW_ERR_NOT_CONVERGED:
+        raise SimulationError(msg)
+    if rc == _abi.CW_ERR_NONFINITE:
+        raise SimulationError(msg)
+    if rc == _abi.CW_ERR_ARG:
+        raise ValueError(msg)
+    raise DeviceError(msg)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def make_opts(omega=0.0, criterion=CRIT_MAX_CHANGE, tolerance=1e-8, max_sweeps=2_000_000,
+              precision=PREC_FP64, sweeps_per_pass=0, check_every=0, inner_tolerance=0.0,
+              max_outer=0) -> RelaxOpts:
+    o = RelaxOpts()
+    _abi.lib().cw_relax_opts_default(C.byref(o))
+    o.omega = omega
+    o.criterion = criterion
+    o.tolerance = tolerance
+    o.max_sweeps = max_sweeps
+    o.precision = precision
+    o.sweeps_per_pass = sweeps_per_pass
+    o.check_every = check_every
+    o.inner_tolerance = inner_tolerance
+    o.max_outer = max_outer
+    return o
+
+
+class Field:
+    def __init__(self, width, height, depth=1, diff=0.2, decay=0.0, initial=0.0):
+        L = _abi.lib()
+        self._h = C.c_void_p()
+        _check(L.cw_field_create(width, height, depth, diff, decay, initial, C.byref(self._h)))
+        self.w, self.h, self.d = width, height, depth
+        self.n = width * height * depth
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _abi.lib().cw_field_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_source(self, source: float):
+        _check(_abi.lib().cw_field_set_source(self._h, source))
+
+    def upload(self, values=None, held=None, uptake=None, production=None):
+        keep = []
+
+        def dbl(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            assert a.size == self.n, (a.size, self.n)
+            keep.append(a)
+            return _dptr(a)
+
+        hp = None
+        if held is not None:
+            hb = np.ascontiguousarray(held, dtype=np.uint8).reshape(-1)
+            assert hb.size == self.n
+            keep.append(hb)
+            hp = hb.ctypes.data_as(C.POINTER(C.c_uint8))
+        _check(_abi.lib().cw_field_upload(self._h, dbl(values), hp, dbl(uptake), dbl(production)))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.float64)
+        _check(_abi.lib().cw_field_download(self._h, _dptr(out)))
+        return out
+
+    def relax(self, tolerance: float, max_sweeps: int) -> int:
+        """ScalarField::relax (pde.rs:245-288): returns the sweep count."""
+        n = C.c_uint64()
+        _check(_abi.lib().cw_field_relax(self._h, tolerance, max_sweeps, C.byref(n)))
+        return int(n.value)
+
+    def relax_ex(self, **kw) -> dict:
+        o = make_opts(**kw)
+        s = RelaxStats()
+        _check(_abi.lib().cw_field_relax_ex(self._h, C.byref(o), C.byref(s)))
+        return s.as_dict()
+
+    def residual(self) -> float:
+        r = C.c_double()
+        _check(_abi.lib().cw_field_residual(self._h, C.byref(r)))
+        return r.value
+
+
+def batch_relax(fields, check=True, **kw):
+    """Solve every field in one launch sequence; returns per-field stats."""
+    o = make_opts(**kw)
+    n = len(fields)
+    arr = (C.c_void_p * n)(*[f._h.value for f in fields])
+    stats = (RelaxStats * n)()
+    rc = _abi.lib().cw_batch_relax(arr, n, C.byref(o), stats)
+    if check:
+        _check(rc)
+    return [s.as_dict() for s in stats], rc
+
+
+__all__ = ["Field", "batch_relax", "make_opts", "CellweaveError", "ConfigError",
+           "SimulationError", "DeviceError", "CRIT_MAX_CHANGE", "CRIT_REL_RESIDUAL",
+           "PREC_FP64", "PREC_FP32_REFINE"]

@@ -1,0 +1,139 @@
+/*
+ * cwfield: C ABI of the MI355X (gfx950) diffusion field solve.
+ *
+ * This is the boundary a thin FFI crate binds so that cellweave's
+ * ScalarField (crates/engine/src/pde.rs:30-53) can delegate its steady-state
+ * solve to the GPU without changing any Rust signature. The host keeps the
+ * canonical arrays; a cw_field handle owns their device mirrors, the
+ * ping-pong state buffers, the reduction scratch and one HIP stream.
+ *
+ * Plain C types only: pointers, sizes, doubles. Every entry point returns a
+ * cw_status; on failure cw_last_error() gives a thread-local message.
+ * Distinct handles may be used from distinct threads at once; one handle is
+ * used by one thread at a time (ScalarField::relax takes &mut self).
+ *
+ * Layout of every per-voxel array: flat index i = (z*H + y)*W + x, x fastest
+ * (pde.rs:27-29, pde.rs:184-188, cpm.rs:194-198).
+ */
+#ifndef CWFIELD_H
+#define CWFIELD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cw_field cw_field;
+
+/* Error codes. CW_ERR_CONFIG maps onto CellweaveError::Config and
+ * CW_ERR_NOT_CONVERGED onto CellweaveError::Simulation
+ * (crates/core/src/error.rs:7-14). */
+typedef enum cw_status {
+    CW_OK = 0,
+    CW_ERR_CONFIG = 1,        /* bad dimensions, non-finite or negative D/decay */
+    CW_ERR_NOT_CONVERGED = 2, /* max_sweeps ran out (pde.rs:285-287) */
+    CW_ERR_NONFINITE = 3,     /* a residual or value turned NaN/inf */
+    CW_ERR_DEVICE = 4,        /* HIP runtime error, or no gfx950 device */
+    CW_ERR_ARG = 5            /* null pointer, size mismatch, bad option */
+} cw_status;
+
+/* Stopping rule. */
+typedef enum cw_criterion {
+    /* The reference rule (pde.rs:279-284): stop after the first sweep in
+     * which no voxel moved by `tolerance` or more. Exact sweep count. */
+    CW_CRIT_MAX_CHANGE = 0,
+    /* New rule: stop when ||rt||_2 / ||b_eff||_2 <= tolerance, where rt is
+     * the projected (natural) residual of the floored problem, tested in
+     * FP64 on the state before a pass. */
+    CW_CRIT_REL_RESIDUAL = 1
+} cw_criterion;
+
+typedef enum cw_precision {
+    CW_PREC_FP64 = 0,        /* plain FP64 red-black projected SOR */
+    CW_PREC_FP32_REFINE = 1  /* FP32 smoothing of the defect equation with
+                                FP64 residual, correction and test */
+} cw_precision;
+
+typedef struct cw_relax_opts {
+    double omega;             /* <= 0: the reference rule (pde.rs:267-277) */
+    int32_t criterion;        /* cw_criterion */
+    double tolerance;         /* absolute max change, or relative residual */
+    uint64_t max_sweeps;      /* hard limit; reaching it is CW_ERR_NOT_CONVERGED */
+    int32_t precision;        /* cw_precision */
+    int32_t sweeps_per_pass;  /* temporal blocking depth s (full sweeps per
+                                 HBM pass); 0 = library choice */
+    int32_t check_every;      /* residual test every k passes; 0 = library choice */
+    double inner_tolerance;   /* FP32_REFINE: inner reduction per outer step; 0 = default */
+    uint64_t max_outer;       /* FP32_REFINE: outer iteration cap; 0 = default */
+} cw_relax_opts;
+
+typedef struct cw_relax_stats {
+    uint64_t sweeps;          /* smoothing sweeps run (FP32_REFINE: inner sweeps summed) */
+    uint64_t passes;          /* sweep kernel launches that did work */
+    uint64_t outer_iters;     /* FP32_REFINE defect-correction steps */
+    uint64_t span_voxels;     /* voxels in the swept box (pde.rs:305-320) */
+    double omega;             /* factor used */
+    double rel_residual;      /* FP64 relative residual at exit (NaN if not computed) */
+    double seconds;           /* wall time of the call */
+    int32_t status;           /* cw_status of this field */
+    int32_t sweeps_per_pass;  /* s actually used */
+} cw_relax_stats;
+
+/* Fill opts with the reference drop-in: reference omega, max-change
+ * criterion, FP64. */
+void cw_relax_opts_default(cw_relax_opts *opts);
+
+/* Thread-local description of the last failure on this thread. */
+const char *cw_last_error(void);
+
+/* Library version string and the device name it runs on. */
+const char *cw_version(void);
+
+/* Replaces ScalarField::with_depth / ScalarField::new (pde.rs:82-131):
+ * same validation, all values `initial`, nothing held, no sinks. */
+cw_status cw_field_create(uint32_t width, uint32_t height, uint32_t depth, double diff,
+                          double decay, double initial, cw_field **out);
+/* Drop. Null is accepted. */
+cw_status cw_field_destroy(cw_field *field);
+
+/* ScalarField::with_source (pde.rs:135-138). */
+cw_status cw_field_set_source(cw_field *field, double source);
+
+/* Bulk upload of the host state before a solve: values (f64), held (one byte
+ * per voxel, nonzero = held, pde.rs:45-46), uptake and production (f64,
+ * pde.rs:47-50). Any pointer may be NULL to keep what the device holds.
+ * Replaces the per-voxel setters (pde.rs:146-220, 468-640) crossing FFI: the
+ * Rust side marks arrays dirty and uploads them here in one call. */
+cw_status cw_field_upload(cw_field *field, const double *values, const uint8_t *held,
+                          const double *uptake, const double *production);
+
+/* Copy the device values back to the host (n doubles). */
+cw_status cw_field_download(const cw_field *field, double *values);
+
+/* ScalarField::relax (pde.rs:245-288) with its exact semantics: the free-span
+ * box, the reference omega, red then black half sweeps, stop on the first
+ * sweep whose largest change is below `tolerance`, Ok(1) when nothing is free,
+ * CW_ERR_NOT_CONVERGED after max_sweeps. *sweeps gets Ok's count. */
+cw_status cw_field_relax(cw_field *field, double tolerance, uint64_t max_sweeps,
+                         uint64_t *sweeps);
+
+/* relax with every knob: omega, criterion, precision, blocking depth. */
+cw_status cw_field_relax_ex(cw_field *field, const cw_relax_opts *opts,
+                            cw_relax_stats *stats);
+
+/* FP64 relative residual (see CW_CRIT_REL_RESIDUAL) of the device values. */
+cw_status cw_field_residual(cw_field *field, double *rel_residual);
+
+/* Batched solve (new; the campaign of src/campaign.rs:196-204 run as one
+ * launch sequence): every field is swept by the same kernel launches, each
+ * stops on its own criterion. stats (may be NULL) has `count` entries.
+ * Returns CW_ERR_NOT_CONVERGED if any field did not converge. */
+cw_status cw_batch_relax(cw_field *const *fields, uint32_t count, const cw_relax_opts *opts,
+                         cw_relax_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
