@@ -1,0 +1,239 @@
+"""Benchmark of the diffusion field solve on one MI355X.
+
+Workload (default): the 2D multi-field case, a 1024^2 plane with ~40k Voronoi
+cells in a radius-450 disc and three fields (oxygen, secreted signal,
+chemokine), each solved from its initial state to an FP64 relative residual
+of 1e-8. One step = the three fields solved together (one batched launch
+sequence) in FP32 smoothing + FP64 defect correction. The plain FP64 solve of
+the same fields is timed beside it.
+
+value = voxel-sweeps/s of the FP32+refine solve: sum over fields of (swept
+voxels x smoothing sweeps), divided by the wall time of the step. Inputs are
+resident in HBM: each step resets the device values with cw_field_fill.
+
+Prints one JSON line. --gpus N with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def optimal_omega(fc):
+    s = fc.span()
+    longest = max(s[1] - s[0] + 1, s[3] - s[2] + 1, s[5] - s[4] + 1)
+    return 2.0 / (1.0 + math.sin(math.pi / max(longest, 2)))
+
+
+def bytes_per_voxel_pass(fc, precision):
+    """Algorithmic HBM bytes per swept voxel per sweep-kernel launch.
+    FP64 plain: read c 8 + write c 8 + folded coefficient 8 = 24.
+    FP32 smoother: read e 4 + write e 4 + defect RHS 4 = 12, plus the
+    correction's lower bound 4 for a field with sinks (the floor can bind)."""
+    if precision == "fp64":
+        return 24.0
+    has_sink = bool((fc.uptake > 0).any())
+    return 16.0 if has_sink else 12.0
+
+
+def cpu_baseline(fc, budget_s=12.0):
+    """The oracle (C restatement of pde.rs, OpenMP over rows in chunks of 4)
+    timed on this host: sweeps of the oxygen field until ~budget_s of work."""
+    from oracle.orf import OracleField
+
+    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
+    f = OracleField(fc.w, fc.h, fc.d, fc.diff, fc.decay, fc.initial)
+    f.values[:] = fc.initial_values()
+    f.copy_next_from_values()
+    f.uptake[:] = fc.uptake
+    f.production[:] = fc.production
+    f.held[:] = fc.held
+    omega = optimal_omega(fc)
+    span = fc.span_voxels()
+    sweeps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        for _ in range(20):
+            f.sor_sweep(omega, threads)
+        sweeps += 20
+    dt = time.perf_counter() - t0
+    return {"value": span * sweeps / dt, "unit": "voxel-sweeps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{sweeps} FP64 red-black sweeps of the {fc.w}x{fc.h} oxygen field "
+                      f"({span} swept voxels), {dt:.1f} s, CPU restatement of pde.rs"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--spp", type=int, default=0, help="FP32 sweeps per pass (0 = library)")
+    ap.add_argument("--spp64", type=int, default=0)
+    ap.add_argument("--inner-tol", type=float, default=0.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--time-launches", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the batched and single solves shard without exchange; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from cellweave_amd import cases
+    from cellweave_amd.field import (CRIT_REL_RESIDUAL, PREC_FP32_REFINE, PREC_FP64, Field,
+                                     batch_relax, time_passes)
+
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only; the solve does not need it
+        sync = lambda: None  # noqa: E731
+
+    log(f"building workload: {args.n}^2 plane, three fields")
+    fcs = cases.config2(n=args.n)
+    fields = []
+    for fc in fcs:
+        g = Field(fc.w, fc.h, fc.d, fc.diff, fc.decay, fc.initial)
+        g.upload(values=fc.initial_values(), held=fc.held, uptake=fc.uptake,
+                 production=fc.production)
+        fields.append(g)
+    omega = optimal_omega(fcs[0])
+    span = [fc.span_voxels() for fc in fcs]
+    tol = 1e-8
+
+    def reset():
+        for g, fc in zip(fields, fcs):
+            g.fill(fc.initial)
+
+    def solve(precision, spp):
+        reset()
+        sync()
+        t0 = time.perf_counter()
+        stats, _ = batch_relax(fields, criterion=CRIT_REL_RESIDUAL, tolerance=tol,
+                               max_sweeps=2_000_000, omega=omega, precision=precision,
+                               sweeps_per_pass=spp, inner_tolerance=args.inner_tol)
+        sync()
+        return time.perf_counter() - t0, stats
+
+    # FP64 plain: one warm-up, then timed solves.
+    log("fp64 warm-up")
+    solve(PREC_FP64, args.spp64)
+    t64s, st64 = [], None
+    for _ in range(max(1, args.steps // 3)):
+        t, st64 = solve(PREC_FP64, args.spp64)
+        t64s.append(t)
+    t64 = float(np.median(t64s))
+    vs64 = sum(s * st["sweeps"] for s, st in zip(span, st64))
+    x64 = [g.download() for g in fields]
+    log(f"fp64: {t64 * 1e3:.2f} ms, sweeps {[st['sweeps'] for st in st64]}, "
+        f"rel {[st['rel_residual'] for st in st64]}")
+
+    log("fp32+refine warm-up")
+    for _ in range(args.warmup):
+        solve(PREC_FP32_REFINE, args.spp)
+    sync()
+    times, st32 = [], None
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t, st32 = solve(PREC_FP32_REFINE, args.spp)
+        times.append(t)
+    sync()
+    total = time.perf_counter() - t_start
+    ms_step = 1e3 * total / args.steps
+    vs32 = sum(s * st["sweeps"] for s, st in zip(span, st32))
+    value = vs32 / (total / args.steps)
+    x32 = [g.download() for g in fields]
+    errs = {}
+    for fc, a, b, st in zip(fcs, x32, x64, st32):
+        scale = float(np.max(np.abs(b))) or 1.0
+        errs[fc.name] = {"max_abs": float(np.max(np.abs(a - b))),
+                         "rel_l2": float(np.linalg.norm(a - b) / (np.linalg.norm(b) or 1.0)),
+                         "max_rel_to_peak": float(np.max(np.abs(a - b)) / scale),
+                         "rel_residual": st["rel_residual"], "sweeps": st["sweeps"],
+                         "outer": st["outer_iters"]}
+    log(f"fp32+refine: {ms_step:.2f} ms/step, sweeps {[st['sweeps'] for st in st32]}, "
+        f"outer {[st['outer_iters'] for st in st32]}")
+
+    # Live kernel timing of the dominant kernel (the FP32 smoother launch).
+    spp_used = st32[0]["sweeps_per_pass"]
+    ms32 = time_passes(fields, args.time_launches, criterion=CRIT_REL_RESIDUAL, tolerance=tol,
+                       omega=omega, precision=PREC_FP32_REFINE, sweeps_per_pass=args.spp)
+    bytes32 = sum(s * bytes_per_voxel_pass(fc, "fp32") for s, fc in zip(span, fcs))
+    achieved = bytes32 / (ms32 * 1e-3) / 1e9
+    reset()
+    ms64 = time_passes(fields, args.time_launches, criterion=CRIT_REL_RESIDUAL, tolerance=tol,
+                       omega=omega, precision=PREC_FP64, sweeps_per_pass=args.spp64)
+    spp64_used = st64[0]["sweeps_per_pass"]
+    bytes64 = sum(s * 24.0 for s in span)
+    achieved64 = bytes64 / (ms64 * 1e-3) / 1e9
+    log(f"kernel: fp32 {ms32 * 1e3:.1f} us/launch ({achieved:.0f} GB/s alg), "
+        f"fp64 {ms64 * 1e3:.1f} us/launch ({achieved64:.0f} GB/s alg)")
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cpu = cpu_baseline(fcs[0])
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_fp32_smoother_config2.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
+    out = {
+        "metric": "voxel-sweeps/s & % HBM roofline, FP32+refine vs FP64 SOR at 1e-8 residual",
+        "value": value,
+        "unit": "voxel-sweeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 smoothing, f64 residual/correction",
+        "data": "synthetic (seeded Voronoi lattice, config 2 of the workload list)",
+        "config": {
+            "workload": f"2D multi-field {args.n}x{args.n}, ~40k cells, oxygen+signal+chemokine, "
+                        "FP32+refine to relative residual 1e-8",
+            "grid": [args.n, args.n], "fields": [fc.name for fc in fcs], "omega": omega,
+            "tolerance": tol, "swept_voxels": span, "sweeps_per_pass": spp_used,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": f"sweep2d<float,{spp_used}> (FP32 smoother, {spp_used} sweeps per launch)",
+            "alg_bytes_per_launch": bytes32, "launch_ms": ms32,
+        },
+        "cpu_baseline": cpu,
+        "fp64": {
+            "voxel_sweeps_per_s": vs64 / t64, "ms_per_solve": t64 * 1e3,
+            "sweeps": [st["sweeps"] for st in st64], "sweeps_per_pass": spp64_used,
+            "kernel_launch_ms": ms64, "kernel_alg_GBs": achieved64,
+            "kernel_frac": achieved64 / HBM_PEAK_GBS,
+        },
+        "fp32_refine": {"ms_per_solve": ms_step, "errors_vs_fp64": errs},
+        "speedup_vs_fp64": {"throughput": value / (vs64 / t64), "time_to_solution": t64 / (ms_step / 1e3)},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

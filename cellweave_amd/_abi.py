@@ -75,6 +75,8 @@ def _declare():
         "cw_field_relax_ex": (st, [P, opts, stats]),
         "cw_field_residual": (st, [P, dp]),
         "cw_batch_relax": (st, [C.POINTER(P), u32, opts, stats]),
+        "cw_field_fill": (st, [P, d]),
+        "cw_batch_time_passes": (st, [C.POINTER(P), u32, opts, u32, dp]),
     })
     del u32p
 

@@ -13,8 +13,16 @@
 // Every launch sweeps a plan: a set of fields ("problems") whose tiles are
 // numbered consecutively; blockIdx -> (problem, tile) via a table, remapped so
 // that each XCD gets a contiguous run of tiles (neighbouring tiles share halo
-// lines in that XCD's L2). Each problem has a device control block (Ctl) that
-// the last workgroup of a launch advances: pass parity, sweep count, status.
+// lines in that XCD's L2).
+//
+// Stopping without fences: launch L of a run finds out by itself whether a
+// problem stopped after launch L-1. Every workgroup of the problem reads what
+// launch L-1 left in memory (the per-sweep largest changes, reduced with
+// atomicMax, and the residual partials of the input) and reaches the same
+// verdict; the problem's first tile records it in the problem's control
+// block. Kernel boundaries make launch L-1's writes visible, so no workgroup
+// ever waits for or fences against another. A one-workgroup `decide` kernel
+// records the verdict on the last launch of a chunk before the host reads it.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,14 +34,13 @@ constexpr int kMaxS = 4;
 constexpr int kWave = 64;
 
 enum Status : int { ST_RUNNING = 0, ST_DONE = 1, ST_MAXED = 2, ST_REPLAY = 3, ST_IDLE = 4 };
-enum Crit : int { CRIT_NONE = 0, CRIT_MAXCHANGE = 1, CRIT_RESID = 2, CRIT_INNER = 3 };
+enum Crit : int { CRIT_NONE = 0, CRIT_MAXCHANGE = 1, CRIT_RESID = 2 };
 enum Mode : int {
-    MODE_SWEEP = 0,     // a pass of s full sweeps; finalize per criterion
-    MODE_REPLAY = 1,    // one sweep for problems rolled back by an exact stop
-    MODE_HALF_RED = 2,  // 3D red half sweep: no finalize
-    MODE_HALF_BLACK = 3,// 3D black half sweep: finalize one sweep
-    MODE_RESID = 4,     // residual pass: r2, b2 per problem
-    MODE_PLAIN = 5      // no finalize (elementwise helpers)
+    MODE_SWEEP = 0,       // a pass of s full sweeps over running problems
+    MODE_REPLAY = 1,      // single sweeps for problems rolled back by an exact stop
+    DECIDE_PASS = 2,      // decide: verdict on the previous pass
+    DECIDE_RESID = 3,     // decide: residual sums after resid64
+    DECIDE_REPLAY_END = 4 // decide: replayed problems are done
 };
 
 // One field in a launch plan. Arrays are pitched: row stride P elements
@@ -41,48 +48,53 @@ enum Mode : int {
 // arrays and NaN in coefficient arrays.
 struct Prob {
     int W, H, D, P;
-    int xs, ys, zs;        // tile-grid origin (xs even): first swept column/row/plane
-    int x1, y1, z1;        // last swept column/row/plane (inclusive)
-    int ntx, nty, ntz;     // sweep tiles (2D: ntz = 1)
-    int sw_base, sw_count; // sweep tiling: first tile, tile count
-    int el_base, el_count; // elementwise tiling (128 x 4 cells per plane)
-    int elx, ely;          // elementwise tiles along x, y
-    int in_place;          // 3D half-sweep kernels: x[0] == x[1]
-    void* x[2];            // state ping-pong (T)
-    const void* k;         // coefficient (T): NaN = held. FP64 plain: production
-                           // minus uptake when at most one is nonzero; else production
-    const void* k2;        // FP64 general form: uptake (else null)
-    const void* lb;        // FP32 inner: lower bound of the correction (or null)
-    double d, decay, omega, omo, source;  // omo = 1 - omega as the host rounds it
-    float inv_den[8];      // FP32: 1/(real*d + decay), real = 0..6
-    // FP64 refinement arrays (FP32_REFINE only)
-    double* c64;           // the FP64 state being corrected
-    float* e0;             // correction buffers of the inner plan (e[0], e[1])
-    float* e1;
-    float* r32;            // FP32 defect right-hand side (NaN = held)
-    float* lb32;           // FP32 lower bound of the correction (or null)
-    int inner_idx;         // this problem's index in the inner plan's Ctl
+    int xs, ys, zs;          // tile-grid origin (xs even): first swept column/row/plane
+    int x1, y1, z1;          // last swept column/row/plane (inclusive)
+    int sw_base[2], sw_count[2], ntx[2], nty[2];  // sweep tilings: [0] s sweeps, [1] one
+    int el_base, el_count, elx, ely;              // elementwise tiling (128 x 4 per plane)
+    int inner_idx;           // this problem's index in the inner plan (refinement)
     int pad_;
+    void* x[2];              // state ping-pong (T); equal for in-place 3D sweeps
+    const void* k;           // coefficient (T): NaN = held. FP64: production minus
+                             // uptake when at most one is nonzero; else production
+    const void* k2;          // FP64 general form: uptake (else null)
+    const void* lb;          // FP32 inner: lower bound of the correction (or null)
+    double d, decay, omega, omo, source;  // omo = 1 - omega as the host rounds it
+    // FP64 refinement arrays
+    double* c64;             // the FP64 state being corrected / tested
+    float* e0;               // correction buffers of the inner plan
+    float* e1;
+    float* r32;              // FP32 defect right-hand side (NaN = held)
+    float* lb32;             // FP32 lower bound of the correction (or null)
 };
 
 struct Ctl {
-    int status, pass, sweeps, replay;
-    unsigned long long moved[kMaxS];
-    double r2, b2, r2_first;
-    int checks, nonfinite;
+    int status;    // Status
+    int parity;    // when stopped: x[parity] holds the answer (replay: the input)
+    int sweeps;    // when stopped: sweeps run
+    int replay;    // REPLAY: single sweeps still to run from x[parity]
+    int budget;    // > 0: stop DONE once this many sweeps have run (inner segments)
+    int nonfinite;
+    int pad0, pad1;
+    unsigned long long moved[3][kMaxS];  // largest change per sweep, ring by pass % 3
+    double r2, b2;                       // last residual sums (squared norms)
 };
 
 struct PassArgs {
     const Prob* probs;
     Ctl* ctl;
     const int* tile_prob;
-    double* partials;   // 2 doubles per tile
-    unsigned* counter;
-    int nprob, ntiles;
-    int crit, check, mode, max_sweeps, s;
-    int tiling;         // 0 sweep tiling, 1 elementwise tiling
+    double* partials;      // 2 ring slots of part_stride doubles (r2, b2 per tile)
+    int part_stride;
+    int nprob;
+    int pass;              // launch index L: running problems have done L passes
+    int sweeps_before;     // sweeps running problems have done before this launch
+    int s, tl;             // this launch: sweeps, sweep tiling
+    int prev_s, prev_tl, prev_check;
+    int check;             // this launch computes the residual of its input
+    int crit, max_sweeps, mode, replay_step;
     double tol;
-    const Ctl* inner_ctl;  // correction kernel: pass parity of the inner plan
+    const Ctl* inner_ctl;  // correct64: the inner plan's control blocks
 };
 
 template <typename T> struct V2;
@@ -112,10 +124,6 @@ __device__ __forceinline__ unsigned long long wave_umax(unsigned long long v) {
     return v;
 }
 
-__device__ __forceinline__ double bits_to_double(unsigned long long b) {
-    return __longlong_as_double((long long)b);
-}
-
 // Largest change of a sweep as the bit pattern of a non-negative double:
 // integer order equals numeric order, so atomicMax is exact and
 // order-independent. NaN changes are dropped, as f64::max drops them.
@@ -123,105 +131,112 @@ __device__ __forceinline__ unsigned long long change_bits(double m) {
     return (m == m) ? (unsigned long long)__double_as_longlong(m) : 0ull;
 }
 
-// ---------------------------------------------------------------------------
-// Finalize: run by every thread of the last workgroup of a launch.
-// ---------------------------------------------------------------------------
-__device__ inline void finalize(const PassArgs& a) {
-    for (int p = threadIdx.x; p < a.nprob; p += blockDim.x) {
-        const Prob& pr = a.probs[p];
-        Ctl& c = a.ctl[p];
-        const int want = (a.mode == MODE_REPLAY) ? ST_REPLAY : ST_RUNNING;
-        if (c.status != want) continue;
-        const int base = a.tiling ? pr.el_base : pr.sw_base;
-        const int cnt = a.tiling ? pr.el_count : pr.sw_count;
-        double r2 = 0.0, b2 = 0.0;
-        if (a.check || a.mode == MODE_RESID) {
-            for (int t = 0; t < cnt; t++) {
-                r2 += __hip_atomic_load(&a.partials[2 * (base + t)], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-                b2 += __hip_atomic_load(&a.partials[2 * (base + t) + 1], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
-            }
-            if (!(r2 == r2) || r2 > 1.7e308) c.nonfinite = 1;
-        }
-        if (a.mode == MODE_RESID) {
-            c.r2 = r2;
-            c.b2 = b2;
-            const double lim = a.tol * a.tol * (b2 > 0.0 ? b2 : 1.0);
-            if (a.crit == CRIT_RESID && r2 <= lim) c.status = ST_DONE;
-            if (c.nonfinite) c.status = ST_MAXED;
-            continue;
-        }
-        if (a.mode == MODE_REPLAY) {
-            c.sweeps += 1;
-            c.pass += 1;
-            c.replay -= 1;
-            if (c.replay <= 0) c.status = ST_DONE;
-            c.moved[0] = 0ull;
-            continue;
-        }
-        bool advance = true;
-        if (a.crit == CRIT_MAXCHANGE) {
-            int found = -1;
-            for (int j = 0; j < a.s; j++) {
-                const unsigned long long mb =
-                    __hip_atomic_load(&c.moved[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (bits_to_double(mb) < a.tol) { found = j; break; }
-            }
-            if (found == a.s - 1) {
-                c.sweeps += a.s;
-                c.pass += 1;
-                c.status = ST_DONE;
-                advance = false;
-            } else if (found >= 0) {
-                // Stopped inside the pass: keep the input (src) and replay
-                // found+1 single sweeps from it.
-                c.status = ST_REPLAY;
-                c.replay = found + 1;
-                advance = false;
-            }
-        } else if (a.crit == CRIT_RESID && a.check) {
-            c.r2 = r2;
-            const double lim = a.tol * a.tol * (c.b2 > 0.0 ? c.b2 : 1.0);
-            if (r2 <= lim) { c.status = ST_DONE; advance = false; }
-        } else if (a.crit == CRIT_INNER && a.check) {
-            if (c.checks == 0) c.r2_first = r2;
-            c.checks += 1;
-            c.r2 = r2;
-            if (r2 <= a.tol * a.tol * c.r2_first) { c.status = ST_DONE; advance = false; }
-        }
-        if (advance) {
-            c.sweeps += a.s;
-            c.pass += 1;
-            if (c.status == ST_RUNNING && c.sweeps >= a.max_sweeps) c.status = ST_MAXED;
-        }
-        if (c.nonfinite && c.status == ST_RUNNING) c.status = ST_MAXED;
-        for (int j = 0; j < kMaxS; j++) c.moved[j] = 0ull;
+// Sum of one problem's tile partials of one ring slot, by one whole wave:
+// lanes stride over the tiles, a fixed shuffle tree adds, so every wave that
+// computes it gets the same bits.
+__device__ __forceinline__ void problem_sums(const double* part, int base, int cnt, double* r2,
+                                             double* b2) {
+    const int lane = threadIdx.x & 63;
+    double r = 0.0, b = 0.0;
+    for (int t = lane; t < cnt; t += 64) {
+        r += part[2 * (base + t)];
+        b += part[2 * (base + t) + 1];
     }
+    *r2 = wave_sum(r);
+    *b2 = wave_sum(b);
 }
 
-// Arrival counter: the last workgroup to arrive runs finalize and re-arms.
-__device__ inline void arrive_and_finalize(const PassArgs& a, bool do_finalize) {
-    __shared__ int s_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned prev = atomicAdd(a.counter, 1u);
-        s_last = (prev == gridDim.x - 1);
+struct Verdict {
+    int status, parity, sweeps, replay, nonfinite;
+};
+
+// Fate of a running problem after launch a.pass - 1.
+__device__ inline Verdict judge(const PassArgs& a, const Ctl& c, bool have_r2, double r2) {
+    Verdict v{ST_RUNNING, a.pass & 1, a.sweeps_before, 0, 0};
+    if (a.pass == 0) return v;
+    const int lp = a.pass - 1;
+    const int sb = a.sweeps_before - a.prev_s;
+    if (have_r2) {
+        if (!(r2 == r2) || r2 > 1.7e308) {
+            v.status = ST_MAXED; v.nonfinite = 1; v.parity = lp & 1; v.sweeps = sb;
+            return v;
+        }
+        if (r2 <= a.tol * a.tol * (c.b2 > 0.0 ? c.b2 : 1.0)) {
+            // converged on the input of launch L-1: that input is the answer
+            v.status = ST_DONE; v.parity = lp & 1; v.sweeps = sb;
+            return v;
+        }
     }
-    __syncthreads();
-    if (s_last) {
-        __threadfence();
-        if (do_finalize) finalize(a);
-        __syncthreads();
+    if (a.crit == CRIT_MAXCHANGE) {
+        int found = -1;
+        for (int j = 0; j < a.prev_s; j++)
+            if (__longlong_as_double((long long)c.moved[lp % 3][j]) < a.tol) { found = j; break; }
+        if (found == a.prev_s - 1) { v.status = ST_DONE; return v; }
+        if (found >= 0) {
+            v.status = ST_REPLAY; v.parity = lp & 1; v.sweeps = sb; v.replay = found + 1;
+            return v;
+        }
+    }
+    if (c.budget > 0 && a.sweeps_before >= c.budget) { v.status = ST_DONE; return v; }
+    if (a.sweeps_before >= a.max_sweeps) v.status = ST_MAXED;
+    return v;
+}
+
+__device__ __forceinline__ void record(Ctl& c, const Verdict& v) {
+    c.parity = v.parity;
+    c.sweeps = v.sweeps;
+    c.replay = v.replay;
+    c.nonfinite = v.nonfinite;
+    c.status = v.status;
+}
+
+// Prologue of every sweep launch: returns whether this workgroup's problem
+// runs in this launch and which buffer holds its input. Thread 0 decides; the
+// problem's first tile records a stop and clears the change slot the next
+// launch will use.
+__device__ inline bool sweep_prologue(const PassArgs& a, const Prob& pr, int p, int lt,
+                                      int* src_parity) {
+    __shared__ int s_run, s_par;
+    Ctl& c = a.ctl[p];
+    if (a.mode == MODE_REPLAY) {
         if (threadIdx.x == 0) {
-            __threadfence();
-            *a.counter = 0u;
+            const bool run = c.status == ST_REPLAY && a.replay_step < c.replay;
+            s_run = run;
+            s_par = (c.parity + a.replay_step) & 1;
         }
+        __syncthreads();
+        *src_parity = s_par;
+        return s_run;
     }
+    const bool have_r2 = a.prev_check && a.crit == CRIT_RESID && a.pass > 0;
+    double r2 = 0.0, b2 = 0.0;
+    if (have_r2 && threadIdx.x < 64) {
+        const int tl = a.prev_tl;
+        problem_sums(a.partials + ((a.pass - 1) & 1) * a.part_stride, pr.sw_base[tl],
+                     pr.sw_count[tl], &r2, &b2);
+    }
+    if (threadIdx.x == 0) {
+        int run = 0;
+        if (c.status == ST_RUNNING) {
+            const Verdict v = judge(a, c, have_r2, r2);
+            run = v.status == ST_RUNNING;
+            if (!run && lt == 0) {
+                if (have_r2) c.r2 = r2;
+                record(c, v);
+            }
+        }
+        if (lt == 0) {
+#pragma unroll
+            for (int j = 0; j < kMaxS; j++) c.moved[(a.pass + 1) % 3][j] = 0ull;
+        }
+        s_run = run;
+    }
+    __syncthreads();
+    *src_parity = a.pass & 1;
+    return s_run;
 }
 
-__device__ __forceinline__ void block_partials(const PassArgs& a, int tile, double r2, double b2) {
+__device__ __forceinline__ void block_partial(double* slot, int tile, double r2, double b2) {
     __shared__ double s_r[16], s_b[16];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     r2 = wave_sum(r2);
@@ -231,8 +246,25 @@ __device__ __forceinline__ void block_partials(const PassArgs& a, int tile, doub
     if (threadIdx.x == 0) {
         double tr = 0.0, tb = 0.0;
         for (int w = 0; w < nw; w++) { tr += s_r[w]; tb += s_b[w]; }
-        a.partials[2 * tile] = tr;
-        a.partials[2 * tile + 1] = tb;
+        slot[2 * tile] = tr;
+        slot[2 * tile + 1] = tb;
+    }
+}
+
+template <int NW, int S>
+__device__ __forceinline__ void block_moved(Ctl& c, int pass, const unsigned long long (&mv)[S]) {
+    __shared__ unsigned long long s_m[NW][S];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < S; j++) {
+        const unsigned long long m = wave_umax(mv[j]);
+        if (lane == 0) s_m[wv][j] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x < S) {
+        unsigned long long m = 0ull;
+        for (int w = 0; w < NW; w++) m = s_m[w][threadIdx.x] > m ? s_m[w][threadIdx.x] : m;
+        if (m) atomicMax(&c.moved[pass % 3][threadIdx.x], m);
     }
 }
 
@@ -240,23 +272,12 @@ __device__ __forceinline__ int real_count(int gx, int gy, int gz, int W, int H, 
     return (gx > 0) + (gx < W - 1) + (gy > 0) + (gy < H - 1) + (gz > 0) + (gz < D - 1);
 }
 
-// ---------------------------------------------------------------------------
-// 2D LDS tile sweep: s full sweeps (2s half sweeps) per launch.
-//
-// A workgroup loads a region 128 wide and RY tall (halo H = 2s around a
-// (128-2H) x (RY-2H) interior) from src into LDS, split by the parity of the
-// local x so every neighbour read is unit-stride across a wave. Wave w owns
-// rows w, w+NW, ...; lane l owns the pair of columns (2l, 2l+1) and keeps its
-// two coefficients per row in registers. Half sweep h updates the cells of
-// colour h&1 at distance > h from the region's edge (the rest is stale halo),
-// so after 2s half sweeps the interior is exact and is stored to dst.
-// ---------------------------------------------------------------------------
 template <typename T> struct Arith;
 template <> struct Arith<double> {
     // Reference order, no contraction (file compiled -ffp-contract=off).
     static __device__ __forceinline__ double balanced(double d, double sum, double k, double k2,
                                                        double source, bool general, int real,
-                                                       double decay, const float*) {
+                                                       double decay) {
         double num = d * sum;
         if (general) {
             num = num + source;
@@ -275,9 +296,9 @@ template <> struct Arith<double> {
 template <> struct Arith<float> {
     // A smoother only: the fast reciprocal (1 ulp) replaces the division.
     static __device__ __forceinline__ float balanced(float d, float sum, float k, float, float,
-                                                     bool, int real, float decay,
-                                                     const float*) {
-        return __builtin_fmaf(d, sum, k) * __builtin_amdgcn_rcpf(__builtin_fmaf((float)real, d, decay));
+                                                     bool, int real, float decay) {
+        return __builtin_fmaf(d, sum, k) *
+               __builtin_amdgcn_rcpf(__builtin_fmaf((float)real, d, decay));
     }
     static __device__ __forceinline__ float relax(float omo, float omega, float before,
                                                   float bal) {
@@ -286,9 +307,24 @@ template <> struct Arith<float> {
 };
 
 template <typename T> __device__ __forceinline__ T nan_of();
-template <> __device__ __forceinline__ double nan_of<double>() { return __longlong_as_double(0x7ff8000000000000ll); }
+template <> __device__ __forceinline__ double nan_of<double>() {
+    return __longlong_as_double(0x7ff8000000000000ll);
+}
 template <> __device__ __forceinline__ float nan_of<float>() { return __int_as_float(0x7fc00000); }
 
+// ---------------------------------------------------------------------------
+// 2D LDS tile sweep: s full sweeps (2s half sweeps) per launch.
+//
+// A workgroup loads a region 128 wide and RY tall (halo H = 2s around a
+// (128-2H) x (RY-2H) interior) from src into LDS, split by the parity of the
+// local x so every neighbour read is unit-stride across a wave. Wave w owns
+// rows w, w+NW, ... (NW even, so all its rows share a parity); lane l owns the
+// pair of columns (2l, 2l+1) and keeps the pair's coefficients in registers,
+// stored by colour so every register index is a compile-time constant. Half
+// sweep h updates the cells of colour h&1 at distance > h from the region's
+// edge (the rest is stale halo), so after 2s half sweeps the interior is exact
+// and is stored to dst.
+// ---------------------------------------------------------------------------
 template <typename T, int S, int RY, int NW, bool PLAIN>
 __global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a) {
     constexpr int H = 2 * S;
@@ -296,7 +332,7 @@ __global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a) {
     constexpr int TX = RX - 2 * H;
     constexpr int TY = RY - 2 * H;
     constexpr int NR = RY / NW;
-    static_assert(RY % NW == 0, "rows split evenly over waves");
+    static_assert(RY % NW == 0 && NW % 2 == 0, "rows split evenly over an even wave count");
     static_assert(TY > 0, "interior rows");
     using T2 = typename V2<T>::type;
     __shared__ T cs[RY][2][64];
@@ -306,166 +342,162 @@ __global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a) {
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int p = a.tile_prob[t];
     const Prob& pr = a.probs[p];
-    const Ctl& ctl = a.ctl[p];
-    const int want = (a.mode == MODE_REPLAY) ? ST_REPLAY : ST_RUNNING;
-    double r2 = 0.0;
-    unsigned long long mv[S];
+    const int tl = a.tl;
+    const int lt = t - pr.sw_base[tl];
+    const int ti = lt % pr.ntx[tl], tj = lt / pr.ntx[tl];
+    const int gxo = pr.xs + ti * TX - H;
+    const int gyo = pr.ys + tj * TY - H;
+    const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
+    const int gz = 0;
+    const int gx = gxo + 2 * lane;
+    const bool colin = gx >= 0 && gx < W;
+    const T* kk = (const T*)pr.k;
+    const T* kk2 = (const T*)pr.k2;
+    const T* lbp = (const T*)pr.lb;
+    const bool general = PLAIN && kk2 != nullptr;
+    const bool has_lb = !PLAIN && lbp != nullptr;
+
+    // Coefficients of the lane's pair, by colour: cb is the colour of the even
+    // column (the same for every row of this wave).
+    const int base = (gxo + gyo + gz) & 1;
+    const int cb = (base + wv) & 1;
+    T k0[NR], k1[NR], u0[NR], u1[NR], l0[NR], l1[NR];
 #pragma unroll
-    for (int j = 0; j < S; j++) mv[j] = 0ull;
+    for (int i = 0; i < NR; i++) {
+        const int gy = gyo + wv + NW * i;
+        T ce = nan_of<T>(), co = nan_of<T>(), ue = T(0), uo = T(0), le = T(0), lo = T(0);
+        if (colin && gy >= 0 && gy < Hh) {
+            const size_t off = (size_t)gy * P + gx;
+            const T2 c = *(const T2*)(kk + off);
+            ce = c.x; co = c.y;
+            if (general) { const T2 u = *(const T2*)(kk2 + off); ue = u.x; uo = u.y; }
+            if (has_lb) { const T2 l = *(const T2*)(lbp + off); le = l.x; lo = l.y; }
+        }
+        k0[i] = cb ? co : ce; k1[i] = cb ? ce : co;
+        u0[i] = cb ? uo : ue; u1[i] = cb ? ue : uo;
+        l0[i] = cb ? lo : le; l1[i] = cb ? le : lo;
+    }
 
-    if (ctl.status == want) {
-        const int lt = t - pr.sw_base;
-        const int ti = lt % pr.ntx, tj = lt / pr.ntx;
-        const int gxo = pr.xs + ti * TX - H;
-        const int gyo = pr.ys + tj * TY - H;
-        const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
-        const int gz = 0;
-        const int base = (gxo + gyo + gz) & 1;
-        const int par = ctl.pass & 1;
-        const T* src = (const T*)pr.x[par];
-        T* dst = (T*)pr.x[par ^ 1];
-        const T* kk = (const T*)pr.k;
-        const T* kk2 = (const T*)pr.k2;
-        const T* lbp = (const T*)pr.lb;
-        const bool general = PLAIN && kk2 != nullptr;
-        const bool has_lb = !PLAIN && lbp != nullptr;
-        const T d = (T)pr.d, decay = (T)pr.decay, omega = (T)pr.omega, omo = (T)pr.omo;
-        const T source = (T)pr.source;
-        const int gx = gxo + 2 * lane;
-        const bool colin = gx >= 0 && gx < W;
+    int par;
+    if (!sweep_prologue(a, pr, p, lt, &par)) return;
+    const T* src = (const T*)pr.x[par];
+    T* dst = (T*)pr.x[par ^ 1];
 
-        T kr[NR][2], k2r[NR][2], lbr[NR][2];
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int y = wv + NW * i;
+        const int gy = gyo + y;
+        T v0 = T(0), v1 = T(0);
+        if (colin && gy >= 0 && gy < Hh) {
+            const T2 v = *(const T2*)(src + (size_t)gy * P + gx);
+            v0 = v.x; v1 = v.y;
+        }
+        cs[y][0][lane] = v0;
+        cs[y][1][lane] = v1;
+    }
+    __syncthreads();
+
+    const T d = (T)pr.d, decay = (T)pr.decay, omega = (T)pr.omega, omo = (T)pr.omo;
+    const T source = (T)pr.source;
+
+    // Residual of the input state on the interior (projected natural residual).
+    double r2 = 0.0;
+    if (a.check) {
 #pragma unroll
         for (int i = 0; i < NR; i++) {
             const int y = wv + NW * i;
-            const int gy = gyo + y;
-            T v0 = T(0), v1 = T(0);
-            T c0 = nan_of<T>(), c1 = nan_of<T>();
-            T u0 = T(0), u1 = T(0), l0 = T(0), l1 = T(0);
-            if (colin && gy >= 0 && gy < Hh) {
-                const size_t off = (size_t)gy * P + gx;
-                const T2 v = *(const T2*)(src + off);
-                const T2 c = *(const T2*)(kk + off);
-                v0 = v.x; v1 = v.y; c0 = c.x; c1 = c.y;
-                if (general) { const T2 u = *(const T2*)(kk2 + off); u0 = u.x; u1 = u.y; }
-                if (has_lb) { const T2 l = *(const T2*)(lbp + off); l0 = l.x; l1 = l.y; }
-            }
-            cs[y][0][lane] = v0;
-            cs[y][1][lane] = v1;
-            kr[i][0] = c0; kr[i][1] = c1;
-            k2r[i][0] = u0; k2r[i][1] = u1;
-            lbr[i][0] = l0; lbr[i][1] = l1;
-        }
-        __syncthreads();
-
-        const float* inv_den = pr.inv_den;
-        // Residual of the input state on the interior (projected natural residual).
-        if (a.check) {
+            if (y < H || y >= H + TY) continue;
 #pragma unroll
-            for (int i = 0; i < NR; i++) {
-                const int y = wv + NW * i;
-                if (y < H || y >= H + TY) continue;
-#pragma unroll
-                for (int q = 0; q < 2; q++) {
-                    const int x = 2 * lane + q;
-                    if (x < H || x >= H + TX) continue;
-                    const T kq = kr[i][q];
-                    if (kq != kq) continue;
-                    const T c = cs[y][q][lane];
-                    const T w = q ? cs[y][0][lane] : cs[y][1][lane - 1];
-                    const T e = q ? cs[y][0][lane + 1] : cs[y][1][lane];
-                    T sum = T(0);
-                    sum += w; sum += e; sum += cs[y - 1][q][lane]; sum += cs[y + 1][q][lane];
-                    const int real = real_count(gxo + x, gyo + y, gz, W, Hh, D);
-                    const double A = (double)real * (double)d + (double)decay;
-                    double b;
-                    if (general) b = ((double)source + (double)kq) - (double)k2r[i][q];
-                    else b = (double)kq;
-                    double r = (b + (double)d * (double)sum) - A * (double)c;
-                    if (PLAIN) {
-                        if (A > 0.0 && !((double)c + r / A > 0.0)) r = -A * (double)c;
-                    } else if (has_lb) {
-                        const double lo = (double)lbr[i][q];
-                        if (A > 0.0 && !((double)c + r / A > lo)) r = A * (lo - (double)c);
-                    }
-                    r2 += r * r;
-                }
-            }
-        }
-
-#pragma unroll
-        for (int h = 0; h < 2 * S; h++) {
-            const int colour = h & 1;
-            const int lo = h + 1, hiY = RY - 2 - h, hiX = RX - 2 - h;
-#pragma unroll
-            for (int i = 0; i < NR; i++) {
-                const int y = wv + NW * i;
-                if (y < lo || y > hiY) continue;
-                const int q = (colour + base + y) & 1;
+            for (int col = 0; col < 2; col++) {
+                const int q = col ^ cb;
                 const int x = 2 * lane + q;
-                if (x < lo || x > hiX) continue;
-                const T kq = q ? kr[i][1] : kr[i][0];
+                if (x < H || x >= H + TX) continue;
+                const T kq = col ? k1[i] : k0[i];
                 if (kq != kq) continue;
-                const T before = cs[y][q][lane];
+                const T c = cs[y][q][lane];
                 const T w = q ? cs[y][0][lane] : cs[y][1][lane - 1];
                 const T e = q ? cs[y][0][lane + 1] : cs[y][1][lane];
                 T sum = T(0);
                 sum += w; sum += e; sum += cs[y - 1][q][lane]; sum += cs[y + 1][q][lane];
                 const int real = real_count(gxo + x, gyo + y, gz, W, Hh, D);
-                const T k2q = q ? k2r[i][1] : k2r[i][0];
-                const T bal = Arith<T>::balanced(d, sum, kq, k2q, source, general, real, decay, inv_den);
-                T after = Arith<T>::relax(omo, omega, before, bal);
+                const double A = (double)real * (double)d + (double)decay;
+                double b;
+                if (general) b = ((double)source + (double)kq) - (double)(col ? u1[i] : u0[i]);
+                else b = (double)kq;
+                double r = (b + (double)d * (double)sum) - A * (double)c;
                 if (PLAIN) {
-                    after = after > T(0) ? after : T(0);
+                    if (A > 0.0 && !((double)c + r / A > 0.0)) r = -A * (double)c;
                 } else if (has_lb) {
-                    const T l = q ? lbr[i][1] : lbr[i][0];
-                    after = after > l ? after : l;
+                    const double lo = (double)(col ? l1[i] : l0[i]);
+                    if (A > 0.0 && !((double)c + r / A > lo)) r = A * (lo - (double)c);
                 }
-                cs[y][q][lane] = after;
-                if (PLAIN && y >= H && y < H + TY && x >= H && x < H + TX) {
-                    const unsigned long long m = change_bits((double)__builtin_fabs(after - before));
-                    mv[h >> 1] = m > mv[h >> 1] ? m : mv[h >> 1];
-                }
-            }
-            __syncthreads();
-        }
-
-        // Store the interior (only rows and pairs that reach the swept box).
-        if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= pr.x1) {
-#pragma unroll
-            for (int i = 0; i < NR; i++) {
-                const int y = wv + NW * i;
-                if (y < H || y >= H + TY) continue;
-                const int gy = gyo + y;
-                if (gy > pr.y1) continue;
-                T2 v;
-                v.x = cs[y][0][lane];
-                v.y = cs[y][1][lane];
-                *(T2*)(dst + (size_t)gy * P + gx) = v;
+                r2 += r * r;
             }
         }
     }
 
-    if (a.check) block_partials(a, t, r2, 0.0);
-    if (PLAIN) {
-        __shared__ unsigned long long s_m[NW][S];
+    unsigned long long mv[S];
 #pragma unroll
-        for (int j = 0; j < S; j++) {
-            const unsigned long long m = wave_umax(mv[j]);
-            if (lane == 0) s_m[wv][j] = m;
+    for (int j = 0; j < S; j++) mv[j] = 0ull;
+#pragma unroll
+    for (int h = 0; h < 2 * S; h++) {
+        const int colour = h & 1;
+        const int lo = h + 1, hiY = RY - 2 - h, hiX = RX - 2 - h;
+        const int q = colour ^ cb;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const int y = wv + NW * i;
+            if (y < lo || y > hiY) continue;
+            const int x = 2 * lane + q;
+            if (x < lo || x > hiX) continue;
+            const T kq = colour ? k1[i] : k0[i];
+            if (kq != kq) continue;
+            const T before = cs[y][q][lane];
+            const T w = q ? cs[y][0][lane] : cs[y][1][lane - 1];
+            const T e = q ? cs[y][0][lane + 1] : cs[y][1][lane];
+            T sum = T(0);
+            sum += w; sum += e; sum += cs[y - 1][q][lane]; sum += cs[y + 1][q][lane];
+            const int real = real_count(gxo + x, gyo + y, gz, W, Hh, D);
+            const T bal = Arith<T>::balanced(d, sum, kq, colour ? u1[i] : u0[i], source, general,
+                                             real, decay);
+            T after = Arith<T>::relax(omo, omega, before, bal);
+            if (PLAIN) {
+                after = after > T(0) ? after : T(0);
+            } else if (has_lb) {
+                const T l = colour ? l1[i] : l0[i];
+                after = after > l ? after : l;
+            }
+            cs[y][q][lane] = after;
+            if (PLAIN && y >= H && y < H + TY && x >= H && x < H + TX) {
+                const unsigned long long m = change_bits((double)__builtin_fabs(after - before));
+                mv[h >> 1] = m > mv[h >> 1] ? m : mv[h >> 1];
+            }
         }
         __syncthreads();
-        if (threadIdx.x < S) {
-            unsigned long long m = 0ull;
-            for (int w = 0; w < NW; w++) m = s_m[w][threadIdx.x] > m ? s_m[w][threadIdx.x] : m;
-            if (m) atomicMax(&a.ctl[p].moved[threadIdx.x], m);
+    }
+
+    // Store the interior (only rows and pairs that reach the swept box).
+    if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= pr.x1) {
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const int y = wv + NW * i;
+            if (y < H || y >= H + TY) continue;
+            const int gy = gyo + y;
+            if (gy > pr.y1) continue;
+            T2 v;
+            v.x = cs[y][0][lane];
+            v.y = cs[y][1][lane];
+            *(T2*)(dst + (size_t)gy * P + gx) = v;
         }
     }
-    arrive_and_finalize(a, true);
+
+    if (a.check) block_partial(a.partials + (a.pass & 1) * a.part_stride, t, r2, 0.0);
+    if (PLAIN && a.crit == CRIT_MAXCHANGE && a.mode == MODE_SWEEP)
+        block_moved<NW, S>(a.ctl[p], a.pass, mv);
 }
 
 // ---------------------------------------------------------------------------
-// Elementwise tiling: 128 cells in x (64 pairs) by NW rows of one plane.
+// Elementwise tiling: 128 cells in x (64 pairs) by 4 rows of one plane.
 // ---------------------------------------------------------------------------
 struct ElTile {
     int gx, gy, gz;   // pair start column, row, plane
@@ -474,14 +506,13 @@ struct ElTile {
 
 __device__ __forceinline__ ElTile el_tile(const Prob& pr, int t) {
     const int lt = t - pr.el_base;
-    const int nw = blockDim.x >> 6;
     const int tx = lt % pr.elx;
     const int rest = lt / pr.elx;
     const int ty = rest % pr.ely;
     const int tz = rest / pr.ely;
     ElTile e;
     e.gx = pr.xs + tx * 128 + 2 * (threadIdx.x & 63);
-    e.gy = pr.ys + ty * nw + (threadIdx.x >> 6);
+    e.gy = pr.ys + ty * 4 + (threadIdx.x >> 6);
     e.gz = pr.zs + tz;
     e.ok = e.gx <= pr.x1 && e.gy <= pr.y1 && e.gz <= pr.z1;
     return e;
@@ -489,134 +520,120 @@ __device__ __forceinline__ ElTile el_tile(const Prob& pr, int t) {
 
 // 3D red or black half sweep in place (x[0] == x[1]): cells of one colour read
 // only the other colour, which this launch never writes, so the result equals
-// the reference's read-old/write-new half sweep (pde.rs:322-328).
+// the reference's read-old/write-new half sweep (pde.rs:322-328). Red and
+// black launches of one pass carry the same pass index.
 template <typename T, bool PLAIN>
 __global__ __launch_bounds__(256) void half3d(PassArgs a, int colour) {
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int p = a.tile_prob[t];
     const Prob& pr = a.probs[p];
-    const Ctl& ctl = a.ctl[p];
-    const int want = (a.mode == MODE_REPLAY) ? ST_REPLAY : ST_RUNNING;
-    unsigned long long mv = 0ull;
-    if (ctl.status == want) {
-        const ElTile et = el_tile(pr, t);
-        if (et.ok) {
-            const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
-            const size_t plane = (size_t)P * Hh;
-            T* xs = (T*)pr.x[0];
-            const T* kk = (const T*)pr.k;
-            const T* kk2 = (const T*)pr.k2;
-            const T* lbp = (const T*)pr.lb;
-            const bool general = PLAIN && kk2 != nullptr;
-            const bool has_lb = !PLAIN && lbp != nullptr;
-            const T d = (T)pr.d, decay = (T)pr.decay, omega = (T)pr.omega, omo = (T)pr.omo;
-            const int q = (colour + et.gx + et.gy + et.gz) & 1;
-            const int x = et.gx + q;
-            if (x < W && x <= pr.x1) {
-                const size_t i = (size_t)et.gz * plane + (size_t)et.gy * P + x;
-                const T kq = kk[i];
-                if (kq == kq) {
-                    const T before = xs[i];
-                    T sum = T(0);
-                    if (x > 0) sum += xs[i - 1];
-                    if (x + 1 < W) sum += xs[i + 1];
-                    if (et.gy > 0) sum += xs[i - P];
-                    if (et.gy + 1 < Hh) sum += xs[i + P];
-                    if (et.gz > 0) sum += xs[i - plane];
-                    if (et.gz + 1 < D) sum += xs[i + plane];
-                    const int real = real_count(x, et.gy, et.gz, W, Hh, D);
-                    const T k2q = general ? kk2[i] : T(0);
-                    const T bal = Arith<T>::balanced(d, sum, kq, k2q, (T)pr.source, general, real,
-                                                     decay, pr.inv_den);
-                    T after = Arith<T>::relax(omo, omega, before, bal);
-                    if (PLAIN) after = after > T(0) ? after : T(0);
-                    else if (has_lb) { const T l = lbp[i]; after = after > l ? after : l; }
-                    xs[i] = after;
-                    if (PLAIN) mv = change_bits((double)__builtin_fabs(after - before));
-                }
+    const int lt = t - pr.el_base;
+    int par;
+    if (!sweep_prologue(a, pr, p, lt, &par)) return;
+    unsigned long long mv[1] = {0ull};
+    const ElTile et = el_tile(pr, t);
+    if (et.ok) {
+        const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
+        const size_t plane = (size_t)P * Hh;
+        T* xs = (T*)pr.x[0];
+        const T* kk = (const T*)pr.k;
+        const T* kk2 = (const T*)pr.k2;
+        const T* lbp = (const T*)pr.lb;
+        const bool general = PLAIN && kk2 != nullptr;
+        const bool has_lb = !PLAIN && lbp != nullptr;
+        const T d = (T)pr.d, decay = (T)pr.decay, omega = (T)pr.omega, omo = (T)pr.omo;
+        const int q = (colour + et.gx + et.gy + et.gz) & 1;
+        const int x = et.gx + q;
+        if (x < W && x <= pr.x1) {
+            const size_t i = (size_t)et.gz * plane + (size_t)et.gy * P + x;
+            const T kq = kk[i];
+            if (kq == kq) {
+                const T before = xs[i];
+                T sum = T(0);
+                if (x > 0) sum += xs[i - 1];
+                if (x + 1 < W) sum += xs[i + 1];
+                if (et.gy > 0) sum += xs[i - P];
+                if (et.gy + 1 < Hh) sum += xs[i + P];
+                if (et.gz > 0) sum += xs[i - plane];
+                if (et.gz + 1 < D) sum += xs[i + plane];
+                const int real = real_count(x, et.gy, et.gz, W, Hh, D);
+                const T k2q = general ? kk2[i] : T(0);
+                const T bal = Arith<T>::balanced(d, sum, kq, k2q, (T)pr.source, general, real,
+                                                 decay);
+                T after = Arith<T>::relax(omo, omega, before, bal);
+                if (PLAIN) after = after > T(0) ? after : T(0);
+                else if (has_lb) { const T l = lbp[i]; after = after > l ? after : l; }
+                xs[i] = after;
+                if (PLAIN) mv[0] = change_bits((double)__builtin_fabs(after - before));
             }
         }
     }
-    if (PLAIN) {
-        __shared__ unsigned long long s_m[4];
-        const unsigned long long m = wave_umax(mv);
-        if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long mm = 0ull;
-            for (int w = 0; w < (int)(blockDim.x >> 6); w++) mm = s_m[w] > mm ? s_m[w] : mm;
-            if (mm) atomicMax(&a.ctl[p].moved[0], mm);
-        }
-    }
-    arrive_and_finalize(a, a.mode == MODE_HALF_BLACK || a.mode == MODE_REPLAY);
+    if (PLAIN && a.crit == CRIT_MAXCHANGE) block_moved<4, 1>(a.ctl[p], a.pass, mv);
 }
 
-// FP64 residual of the current state (2D and 3D), elementwise tiling.
-// Per free cell: b = (source + production) - uptake (or the folded k),
-// r = (b + d*sum) - A*c, projected to the natural residual; b_eff adds the
-// held neighbours. With write32 it also writes the FP32 defect right-hand
-// side r, the correction's lower bound -c (rounded toward zero) and zeroes
-// e[0] for the FP32 inner solve.
+// FP64 residual of the current state (2D and 3D), elementwise tiling, into
+// ring slot 0 of the partials. Per free cell: b = (source + production) -
+// uptake (or the folded k), r = (b + d*sum) - A*c, projected to the natural
+// residual; b_eff adds the held neighbours. With write32 it also writes the
+// FP32 defect right-hand side r, the correction's lower bound -c (rounded
+// toward zero) and zeroes e[0] for the FP32 inner solve.
 __global__ __launch_bounds__(256) void resid64(PassArgs a, int write32) {
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int p = a.tile_prob[t];
     const Prob& pr = a.probs[p];
-    const Ctl& ctl = a.ctl[p];
     double r2 = 0.0, b2 = 0.0;
-    if (ctl.status == ST_RUNNING) {
-        const ElTile et = el_tile(pr, t);
-        if (et.ok) {
-            const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
-            const size_t plane = (size_t)P * Hh;
-            const double* c = pr.c64;
-            const double* kk = (const double*)pr.k;
-            const double* kk2 = (const double*)pr.k2;
-            const double d = pr.d, decay = pr.decay;
+    if (a.ctl[p].status != ST_RUNNING) return;
+    const ElTile et = el_tile(pr, t);
+    if (et.ok) {
+        const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
+        const size_t plane = (size_t)P * Hh;
+        const double* c = pr.c64;
+        const double* kk = (const double*)pr.k;
+        const double* kk2 = (const double*)pr.k2;
+        const double d = pr.d, decay = pr.decay;
 #pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const int x = et.gx + q;
-                if (x >= W) continue;
-                const size_t i = (size_t)et.gz * plane + (size_t)et.gy * P + x;
-                const double kq = kk[i];
-                if (kq != kq) {
-                    if (write32) { pr.r32[i] = __int_as_float(0x7fc00000); pr.e0[i] = 0.0f; }
-                    continue;
-                }
-                const double ci = c[i];
-                double sum = 0.0, hsum = 0.0;
-                int real = 0;
-#define CW_NB(cond, j)                                   \
-    if (cond) {                                          \
-        const double v = c[j];                           \
-        sum += v;                                        \
-        real += 1;                                       \
-        if (kk[j] != kk[j]) hsum += v;                   \
+        for (int q = 0; q < 2; q++) {
+            const int x = et.gx + q;
+            if (x >= W) continue;
+            const size_t i = (size_t)et.gz * plane + (size_t)et.gy * P + x;
+            const double kq = kk[i];
+            if (kq != kq) {
+                if (write32) { pr.r32[i] = __int_as_float(0x7fc00000); pr.e0[i] = 0.0f; }
+                continue;
+            }
+            const double ci = c[i];
+            double sum = 0.0, hsum = 0.0;
+            int real = 0;
+#define CW_NB(cond, j)                 \
+    if (cond) {                        \
+        const double v = c[j];         \
+        sum += v;                      \
+        real += 1;                     \
+        if (kk[j] != kk[j]) hsum += v; \
     }
-                CW_NB(x > 0, i - 1)
-                CW_NB(x + 1 < W, i + 1)
-                CW_NB(et.gy > 0, i - P)
-                CW_NB(et.gy + 1 < Hh, i + P)
-                CW_NB(et.gz > 0, i - plane)
-                CW_NB(et.gz + 1 < D, i + plane)
+            CW_NB(x > 0, i - 1)
+            CW_NB(x + 1 < W, i + 1)
+            CW_NB(et.gy > 0, i - P)
+            CW_NB(et.gy + 1 < Hh, i + P)
+            CW_NB(et.gz > 0, i - plane)
+            CW_NB(et.gz + 1 < D, i + plane)
 #undef CW_NB
-                const double A = (double)real * d + decay;
-                const double b = kk2 ? ((pr.source + kq) - kk2[i]) : kq;
-                const double r = (b + d * sum) - A * ci;
-                double rt = r;
-                if (A > 0.0 && !(ci + r / A > 0.0)) rt = -A * ci;
-                const double be = b + d * hsum;
-                r2 += rt * rt;
-                b2 += be * be;
-                if (write32) {
-                    pr.r32[i] = (float)r;
-                    pr.e0[i] = 0.0f;
-                    if (pr.lb32) pr.lb32[i] = __double2float_rz(-ci);
-                }
+            const double A = (double)real * d + decay;
+            const double b = kk2 ? ((pr.source + kq) - kk2[i]) : kq;
+            const double r = (b + d * sum) - A * ci;
+            double rt = r;
+            if (A > 0.0 && !(ci + r / A > 0.0)) rt = -A * ci;
+            const double be = b + d * hsum;
+            r2 += rt * rt;
+            b2 += be * be;
+            if (write32) {
+                pr.r32[i] = (float)r;
+                pr.e0[i] = 0.0f;
+                if (pr.lb32) pr.lb32[i] = __double2float_rz(-ci);
             }
         }
     }
-    block_partials(a, t, r2, b2);
-    arrive_and_finalize(a, true);
+    block_partial(a.partials, t, r2, b2);
 }
 
 // c <- max(c + e, 0) on the free cells, e from the buffer the inner plan ended in.
@@ -624,27 +641,86 @@ __global__ __launch_bounds__(256) void correct64(PassArgs a) {
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int p = a.tile_prob[t];
     const Prob& pr = a.probs[p];
-    const Ctl& ctl = a.ctl[p];
-    if (ctl.status == ST_RUNNING) {
-        const ElTile et = el_tile(pr, t);
-        if (et.ok) {
-            const int W = pr.W, P = pr.P;
-            const size_t plane = (size_t)P * pr.H;
-            const Ctl& ic = a.inner_ctl[pr.inner_idx];
-            const float* e = (ic.pass & 1) ? pr.e1 : pr.e0;
-            const double* kk = (const double*)pr.k;
+    if (a.ctl[p].status != ST_RUNNING) return;
+    const ElTile et = el_tile(pr, t);
+    if (!et.ok) return;
+    const int W = pr.W, P = pr.P;
+    const size_t plane = (size_t)P * pr.H;
+    const Ctl& ic = a.inner_ctl[pr.inner_idx];
+    const float* e = (ic.parity & 1) ? pr.e1 : pr.e0;
+    const double* kk = (const double*)pr.k;
 #pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const int x = et.gx + q;
-                if (x >= W) continue;
-                const size_t i = (size_t)et.gz * plane + (size_t)et.gy * P + x;
-                if (kk[i] != kk[i]) continue;
-                const double v = pr.c64[i] + (double)e[i];
-                pr.c64[i] = v > 0.0 ? v : 0.0;
+    for (int q = 0; q < 2; q++) {
+        const int x = et.gx + q;
+        if (x >= W) continue;
+        const size_t i = (size_t)et.gz * plane + (size_t)et.gy * P + x;
+        if (kk[i] != kk[i]) continue;
+        const double v = pr.c64[i] + (double)e[i];
+        pr.c64[i] = v > 0.0 ? v : 0.0;
+    }
+}
+
+// One workgroup: records verdicts for the host. DECIDE_PASS judges the last
+// launch of a chunk (as the next sweep launch would); DECIDE_RESID turns the
+// resid64 partials into r2, b2 and stops converged problems; DECIDE_REPLAY_END
+// closes replayed problems.
+__global__ __launch_bounds__(256) void decide(PassArgs a) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int p = wv; p < a.nprob; p += nw) {
+        const Prob& pr = a.probs[p];
+        Ctl& c = a.ctl[p];
+        if (a.mode == DECIDE_REPLAY_END) {
+            if (lane == 0 && c.status == ST_REPLAY) {
+                c.sweeps += c.replay;
+                c.parity = (c.parity + c.replay) & 1;
+                c.replay = 0;
+                c.status = ST_DONE;
+            }
+            continue;
+        }
+        if (c.status != ST_RUNNING) continue;
+        if (a.mode == DECIDE_RESID) {
+            double r2, b2;
+            problem_sums(a.partials, pr.el_base, pr.el_count, &r2, &b2);
+            if (lane == 0) {
+                c.r2 = r2;
+                c.b2 = b2;
+                if (!(r2 == r2) || r2 > 1.7e308) {
+                    c.nonfinite = 1;
+                    c.status = ST_MAXED;
+                    c.sweeps = a.sweeps_before;
+                } else if (a.crit == CRIT_RESID &&
+                           r2 <= a.tol * a.tol * (b2 > 0.0 ? b2 : 1.0)) {
+                    c.status = ST_DONE;
+                    c.sweeps = a.sweeps_before;
+                    c.parity = a.pass & 1;
+                }
+            }
+            continue;
+        }
+        const bool have_r2 = a.prev_check && a.crit == CRIT_RESID && a.pass > 0;
+        double r2 = 0.0, b2 = 0.0;
+        if (have_r2)
+            problem_sums(a.partials + ((a.pass - 1) & 1) * a.part_stride, pr.sw_base[a.prev_tl],
+                         pr.sw_count[a.prev_tl], &r2, &b2);
+        if (lane == 0) {
+            const Verdict v = judge(a, c, have_r2, r2);
+            if (v.status != ST_RUNNING) {
+                if (have_r2) c.r2 = r2;
+                record(c, v);
             }
         }
     }
-    arrive_and_finalize(a, false);
+}
+
+// Every cell of the box (not the row padding) set to v: ScalarField::fill
+// (pde.rs:212-215) on the device.
+__global__ __launch_bounds__(256) void fill_cells(double* c, int W, long rows, int P, double v) {
+    const long n = rows * (long)W;
+    for (long i = blockIdx.x * 256l + threadIdx.x; i < n; i += (long)gridDim.x * 256l) {
+        const long r = i / W, x = i - r * W;
+        c[r * P + x] = v;
+    }
 }
 
 }  // namespace cw

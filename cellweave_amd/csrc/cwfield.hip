@@ -214,6 +214,7 @@ struct Plan {
     int nprob = 0;
     int s_main = 1;
     bool fp64 = true;
+    bool plain = true;   // FP64: floored field (false: the unfloored adjoint)
     bool three_d = false;
     std::vector<Prob> hp;
     std::vector<Ctl> hc;
@@ -222,26 +223,21 @@ struct Plan {
     Ctl* dc = nullptr;
     int* dtp[3] = {nullptr, nullptr, nullptr};
     double* partials = nullptr;
-    unsigned* counter = nullptr;
-    size_t cap_prob = 0, cap_tp[3] = {0, 0, 0}, cap_part = 0;
-    // per problem, per tiling: base/count/ntx/nty
-    std::vector<int> base[3], count[3], ntx[3], nty[3];
+    int part_stride = 0;
+    size_t cap_prob = 0, cap_ctl = 0, cap_tp[3] = {0, 0, 0}, cap_part = 0;
+    // host mirror of the pass sequence of the current run
+    int pass = 0, sweeps_before = 0, prev_s = 0, prev_tl = 0, prev_check = 0;
 
-    ~Plan() { release(); }
-    void release() {
-        hipFree(dp); hipFree(dc); hipFree(partials); hipFree(counter);
-        for (int i = 0; i < 3; i++) hipFree(dtp[i]);
-        dp = nullptr; dc = nullptr; partials = nullptr; counter = nullptr;
-        for (int i = 0; i < 3; i++) dtp[i] = nullptr;
-        cap_prob = cap_part = 0;
-        for (int i = 0; i < 3; i++) cap_tp[i] = 0;
+    ~Plan() {
+        (void)hipFree(dp); (void)hipFree(dc); (void)hipFree(partials);
+        for (int i = 0; i < 3; i++) (void)hipFree(dtp[i]);
     }
 };
 
 template <typename T>
 cw_status grow(T** p, size_t* cap, size_t need) {
     if (need <= *cap) return CW_OK;
-    hipFree(*p);
+    (void)hipFree(*p);
     *p = nullptr;
     size_t c = std::max<size_t>(need, 2 * *cap);
     CW_HIP(hipMalloc((void**)p, c * sizeof(T)));
@@ -249,15 +245,11 @@ cw_status grow(T** p, size_t* cap, size_t need) {
     return CW_OK;
 }
 
-// Lays out the tilings of every problem (main sweep tile, one-sweep tile,
+// Lays out the tilings of every problem (s-sweep tile, one-sweep tile,
 // elementwise tile) and uploads problems, control blocks and tables.
 cw_status plan_upload(Plan& pl) {
     const int np = pl.nprob;
-    for (int tl = 0; tl < 3; tl++) {
-        pl.base[tl].assign(np, 0); pl.count[tl].assign(np, 0);
-        pl.ntx[tl].assign(np, 0); pl.nty[tl].assign(np, 0);
-        pl.tp[tl].clear();
-    }
+    for (int tl = 0; tl < 3; tl++) pl.tp[tl].clear();
     for (int p = 0; p < np; p++) {
         Prob& pr = pl.hp[p];
         const int sx = pr.x1 - pr.xs + 1, sy = pr.y1 - pr.ys + 1, sz = pr.z1 - pr.zs + 1;
@@ -265,21 +257,24 @@ cw_status plan_upload(Plan& pl) {
             int nx, ny, nz;
             if (tl == TL_EL) {
                 nx = (sx + 127) / 128; ny = (sy + 3) / 4; nz = sz;
+            } else if (pl.three_d) {
+                nx = ny = nz = 0;
             } else {
                 const TileShape ts = sweep_shape(pl.fp64, tl == TL_MAIN ? pl.s_main : 1);
                 nx = (sx + ts.tx - 1) / ts.tx; ny = (sy + ts.ty - 1) / ts.ty; nz = 1;
-                if (pl.three_d) { nx = ny = nz = 0; }
             }
-            pl.base[tl][p] = (int)pl.tp[tl].size();
-            pl.count[tl][p] = nx * ny * nz;
-            pl.ntx[tl][p] = nx; pl.nty[tl][p] = ny;
-            pl.tp[tl].insert(pl.tp[tl].end(), (size_t)nx * ny * nz, p);
+            const int base = (int)pl.tp[tl].size(), count = nx * ny * nz;
+            if (tl == TL_EL) {
+                pr.el_base = base; pr.el_count = count; pr.elx = nx; pr.ely = ny;
+            } else {
+                pr.sw_base[tl] = base; pr.sw_count[tl] = count; pr.ntx[tl] = nx; pr.nty[tl] = ny;
+            }
+            pl.tp[tl].insert(pl.tp[tl].end(), (size_t)count, p);
         }
     }
     cw_status st;
-    size_t cap_ctl = pl.cap_prob;
     if ((st = grow(&pl.dp, &pl.cap_prob, (size_t)std::max(1, np)))) return st;
-    if ((st = grow(&pl.dc, &cap_ctl, (size_t)std::max(1, np)))) return st;
+    if ((st = grow(&pl.dc, &pl.cap_ctl, (size_t)std::max(1, np)))) return st;
     size_t maxt = 1;
     for (int tl = 0; tl < 3; tl++) {
         maxt = std::max(maxt, pl.tp[tl].size());
@@ -288,63 +283,46 @@ cw_status plan_upload(Plan& pl) {
         CW_HIP(hipMemcpyAsync(pl.dtp[tl], pl.tp[tl].data(), pl.tp[tl].size() * sizeof(int),
                               hipMemcpyHostToDevice, pl.stream));
     }
-    if ((st = grow(&pl.partials, &pl.cap_part, 2 * maxt))) return st;
-    if (!pl.counter) {
-        CW_HIP(hipMalloc((void**)&pl.counter, sizeof(unsigned)));
-    }
-    CW_HIP(hipMemsetAsync(pl.counter, 0, sizeof(unsigned), pl.stream));
+    pl.part_stride = (int)(2 * maxt);
+    if ((st = grow(&pl.partials, &pl.cap_part, 2 * (size_t)pl.part_stride))) return st;
+    CW_HIP(hipMemcpyAsync(pl.dp, pl.hp.data(), np * sizeof(Prob), hipMemcpyHostToDevice,
+                          pl.stream));
     CW_HIP(hipMemcpyAsync(pl.dc, pl.hc.data(), np * sizeof(Ctl), hipMemcpyHostToDevice, pl.stream));
     return CW_OK;
 }
 
-// Problem descriptors carry one tiling at a time (the kernels read
-// sw_base/ntx for sweeps, el_base/elx for elementwise work).
-cw_status plan_set_tiling(Plan& pl, int tl) {
-    for (int p = 0; p < pl.nprob; p++) {
-        Prob& pr = pl.hp[p];
-        if (tl != TL_EL) {
-            pr.sw_base = pl.base[tl][p]; pr.sw_count = pl.count[tl][p];
-            pr.ntx = pl.ntx[tl][p]; pr.nty = pl.nty[tl][p]; pr.ntz = 1;
-        }
-        pr.el_base = pl.base[TL_EL][p]; pr.el_count = pl.count[TL_EL][p];
-        pr.elx = pl.ntx[TL_EL][p]; pr.ely = pl.nty[TL_EL][p];
-    }
-    CW_HIP(hipMemcpyAsync(pl.dp, pl.hp.data(), pl.nprob * sizeof(Prob), hipMemcpyHostToDevice,
-                          pl.stream));
-    return CW_OK;
-}
-
-PassArgs make_args(Plan& pl, int tl, int mode, int crit, int check, int s, double tol,
-                   int max_sweeps) {
+PassArgs make_args(Plan& pl, int tl, int mode, int crit, double tol, int max_sweeps) {
     PassArgs a{};
     a.probs = pl.dp;
     a.ctl = pl.dc;
     a.tile_prob = pl.dtp[tl];
     a.partials = pl.partials;
-    a.counter = pl.counter;
+    a.part_stride = pl.part_stride;
     a.nprob = pl.nprob;
-    a.ntiles = (int)pl.tp[tl].size();
+    a.pass = pl.pass;
+    a.sweeps_before = pl.sweeps_before;
+    a.prev_s = pl.prev_s;
+    a.prev_tl = pl.prev_tl;
+    a.prev_check = pl.prev_check;
+    a.tl = tl == TL_EL ? 0 : tl;
     a.crit = crit;
-    a.check = check;
     a.mode = mode;
     a.max_sweeps = max_sweeps;
-    a.s = s;
-    a.tiling = (tl == TL_EL) ? 1 : 0;
     a.tol = tol;
-    a.inner_ctl = nullptr;
     return a;
 }
 
-cw_status launch_sweep(Plan& pl, int tl, int s, bool plain, const PassArgs& a) {
-    const int nt = (int)pl.tp[tl].size();
+cw_status launch_sweep(Plan& pl, const PassArgs& a) {
+    const int nt = (int)pl.tp[a.tl].size();
     if (nt == 0) return CW_OK;
+    const int s = a.s;
     dim3 grid(nt);
     if (pl.fp64) {
         if (s == 1) {
-            if (plain) hipLaunchKernelGGL((sweep2d<double, 1, 32, 4, true>), grid, dim3(256), 0, pl.stream, a);
+            if (pl.plain) hipLaunchKernelGGL((sweep2d<double, 1, 32, 4, true>), grid, dim3(256), 0, pl.stream, a);
             else hipLaunchKernelGGL((sweep2d<double, 1, 32, 4, false>), grid, dim3(256), 0, pl.stream, a);
         } else {
-            if (plain) hipLaunchKernelGGL((sweep2d<double, 2, 32, 4, true>), grid, dim3(256), 0, pl.stream, a);
+            if (pl.plain) hipLaunchKernelGGL((sweep2d<double, 2, 32, 4, true>), grid, dim3(256), 0, pl.stream, a);
             else hipLaunchKernelGGL((sweep2d<double, 2, 32, 4, false>), grid, dim3(256), 0, pl.stream, a);
         }
     } else {
@@ -356,17 +334,20 @@ cw_status launch_sweep(Plan& pl, int tl, int s, bool plain, const PassArgs& a) {
     return CW_OK;
 }
 
-cw_status launch_half3d(Plan& pl, bool plain, PassArgs a, int colour) {
+cw_status launch_half3d(Plan& pl, PassArgs a, int colour) {
     const int nt = (int)pl.tp[TL_EL].size();
     if (nt == 0) return CW_OK;
     a.tile_prob = pl.dtp[TL_EL];
-    a.tiling = 1;
-    if (pl.fp64) {
-        if (plain) hipLaunchKernelGGL((half3d<double, true>), dim3(nt), dim3(256), 0, pl.stream, a, colour);
-        else hipLaunchKernelGGL((half3d<double, false>), dim3(nt), dim3(256), 0, pl.stream, a, colour);
-    } else {
-        hipLaunchKernelGGL((half3d<float, false>), dim3(nt), dim3(256), 0, pl.stream, a, colour);
-    }
+    if (pl.fp64 && pl.plain) hipLaunchKernelGGL((half3d<double, true>), dim3(nt), dim3(256), 0, pl.stream, a, colour);
+    else if (pl.fp64) hipLaunchKernelGGL((half3d<double, false>), dim3(nt), dim3(256), 0, pl.stream, a, colour);
+    else hipLaunchKernelGGL((half3d<float, false>), dim3(nt), dim3(256), 0, pl.stream, a, colour);
+    CW_HIP(hipGetLastError());
+    return CW_OK;
+}
+
+cw_status launch_decide(Plan& pl, int mode, int crit, double tol, int max_sweeps) {
+    PassArgs a = make_args(pl, TL_MAIN, mode, crit, tol, max_sweeps);
+    hipLaunchKernelGGL(decide, dim3(1), dim3(256), 0, pl.stream, a);
     CW_HIP(hipGetLastError());
     return CW_OK;
 }
@@ -390,73 +371,91 @@ bool any_status(const Plan& pl, int st) {
     return false;
 }
 
+void run_reset(Plan& pl) {
+    pl.pass = 0; pl.sweeps_before = 0; pl.prev_s = 0; pl.prev_tl = 0; pl.prev_check = 0;
+}
+
+// One sweep launch (2D: s sweeps; 3D: a red and a black half sweep).
+cw_status one_pass(Plan& pl, int s, int crit, int check, double tol, int max_sweeps) {
+    cw_status st;
+    if (pl.three_d) {
+        PassArgs a = make_args(pl, TL_EL, MODE_SWEEP, crit, tol, max_sweeps);
+        a.s = 1;
+        if ((st = launch_half3d(pl, a, 0))) return st;
+        if ((st = launch_half3d(pl, a, 1))) return st;
+        s = 1;
+        check = 0;
+    } else {
+        const int tl = (s == pl.s_main) ? TL_MAIN : TL_ONE;
+        PassArgs a = make_args(pl, tl, MODE_SWEEP, crit, tol, max_sweeps);
+        a.s = s;
+        a.check = check;
+        if ((st = launch_sweep(pl, a))) return st;
+        pl.prev_tl = tl;
+    }
+    pl.pass += 1;
+    pl.sweeps_before += s;
+    pl.prev_s = s;
+    pl.prev_check = check;
+    return CW_OK;
+}
+
 struct LoopResult { long passes = 0; };
 
-// Runs passes of the plan until no problem is RUNNING or `max_sweeps` sweeps
-// have run in this call; the device marks a problem MAXED once its own count
-// reaches dev_max. Rolls back and replays exact stops inside a pass.
-cw_status run_passes(Plan& pl, bool plain, int crit, double tol, uint64_t max_sweeps,
-                     uint64_t dev_max, int check_every, LoopResult* out) {
-    const int s = pl.s_main;
-    uint64_t host_sweeps = 0;
-    long pass_idx = 0;
-    int chunk = 4;
+// Runs passes until no problem is RUNNING or `budget` sweeps have run; the
+// device stops a problem once the run has done `dev_max` sweeps. Problems
+// that stopped inside a fused pass are replayed one sweep at a time from the
+// pass's input.
+cw_status run_passes(Plan& pl, int crit, double tol, uint64_t budget, uint64_t dev_max,
+                     int check_every, LoopResult* out) {
     cw_status st;
     const int maxs = (int)std::min<uint64_t>(dev_max, 0x7fffffff);
+    const uint64_t start = (uint64_t)pl.sweeps_before;
+    long launched = 0;
+    int chunk = 4;
     while (true) {
         for (int j = 0; j < chunk; j++) {
-            if (host_sweeps >= max_sweeps) break;
-            const uint64_t left = max_sweeps - host_sweeps;
-            const int check = (crit == CRIT_RESID || crit == CRIT_INNER)
-                                  ? (pass_idx % check_every == 0)
-                                  : 0;
-            if (pl.three_d) {
-                PassArgs a = make_args(pl, TL_EL, MODE_HALF_RED, crit, 0, 1, tol, maxs);
-                if ((st = launch_half3d(pl, plain, a, 0))) return st;
-                a.mode = MODE_HALF_BLACK;
-                if ((st = launch_half3d(pl, plain, a, 1))) return st;
-                host_sweeps += 1;
-            } else if ((uint64_t)s <= left) {
-                PassArgs a = make_args(pl, TL_MAIN, MODE_SWEEP, crit, check, s, tol, maxs);
-                if ((st = launch_sweep(pl, TL_MAIN, s, plain, a))) return st;
-                host_sweeps += s;
-            } else {
-                if ((st = plan_set_tiling(pl, TL_ONE))) return st;
-                PassArgs a = make_args(pl, TL_ONE, MODE_SWEEP, crit, check, 1, tol, maxs);
-                if ((st = launch_sweep(pl, TL_ONE, 1, plain, a))) return st;
-                if ((st = plan_set_tiling(pl, TL_MAIN))) return st;
-                host_sweeps += 1;
-            }
-            pass_idx++;
+            const uint64_t done = (uint64_t)pl.sweeps_before - start;
+            if (done >= budget) break;
+            const int s = (int)std::min<uint64_t>((uint64_t)pl.s_main, budget - done);
+            const int check = crit == CRIT_RESID && (pl.pass % check_every == 0);
+            if ((st = one_pass(pl, s, crit, check, tol, maxs))) return st;
+            launched++;
         }
+        if ((st = launch_decide(pl, DECIDE_PASS, crit, tol, maxs))) return st;
         if ((st = read_ctl(pl))) return st;
-        if (!any_status(pl, ST_RUNNING) || host_sweeps >= max_sweeps) break;
+        if (!any_status(pl, ST_RUNNING)) break;
+        if ((uint64_t)pl.sweeps_before - start >= budget) break;
         chunk = std::min(chunk * 2, 64);
     }
-    // Exact stop inside a fused pass: replay single sweeps from the kept input.
     if (any_status(pl, ST_REPLAY)) {
         int most = 0;
         for (int p = 0; p < pl.nprob; p++)
             if (pl.hc[p].status == ST_REPLAY) most = std::max(most, pl.hc[p].replay);
-        if ((st = plan_set_tiling(pl, TL_ONE))) return st;
         for (int r = 0; r < most; r++) {
-            PassArgs a = make_args(pl, TL_ONE, MODE_REPLAY, crit, 0, 1, tol, maxs);
-            if ((st = launch_sweep(pl, TL_ONE, 1, plain, a))) return st;
+            PassArgs a = make_args(pl, TL_ONE, MODE_REPLAY, crit, tol, maxs);
+            a.s = 1;
+            a.replay_step = r;
+            if ((st = launch_sweep(pl, a))) return st;
         }
-        if ((st = plan_set_tiling(pl, TL_MAIN))) return st;
+        if ((st = launch_decide(pl, DECIDE_REPLAY_END, crit, tol, maxs))) return st;
         if ((st = read_ctl(pl))) return st;
     }
-    if (out) out->passes = pass_idx;
+    if (out) out->passes += launched;
     return CW_OK;
 }
 
+// FP64 residual of every running problem (optionally writing the FP32
+// defect inputs), then r2/b2 recorded and converged problems stopped.
 cw_status run_resid(Plan& pl, int crit, double tol, int write32) {
     const int nt = (int)pl.tp[TL_EL].size();
     if (nt == 0) return CW_OK;
-    PassArgs a = make_args(pl, TL_EL, MODE_RESID, crit, 1, 1, tol, 0);
+    PassArgs a = make_args(pl, TL_EL, DECIDE_RESID, crit, tol, 0);
     hipLaunchKernelGGL(resid64, dim3(nt), dim3(256), 0, pl.stream, a, write32);
     CW_HIP(hipGetLastError());
-    return CW_OK;
+    cw_status st = launch_decide(pl, DECIDE_RESID, crit, tol, 0);
+    if (st) return st;
+    return read_ctl(pl);
 }
 
 void fill_prob_common(Prob& pr, const cw_field* f, double omega) {
@@ -466,7 +465,6 @@ void fill_prob_common(Prob& pr, const cw_field* f, double omega) {
     pr.x1 = f->span[1]; pr.y1 = f->span[3]; pr.z1 = f->span[5];
     pr.d = f->diff; pr.decay = f->decay; pr.source = f->source;
     pr.omega = omega; pr.omo = 1.0 - omega;
-    for (int r = 0; r < 8; r++) pr.inv_den[r] = (float)(1.0 / ((double)r * f->diff + f->decay));
 }
 
 void init_ctl(Ctl& c, int status) {
@@ -484,7 +482,31 @@ struct Opts {
     int check_every;
     double inner_tol;
     uint64_t max_outer;
+    long time_passes = 0;      // > 0: time this many sweep launches instead of solving
+    double* time_ms = nullptr; // average duration of one sweep launch (HIP events)
 };
+
+// Times n sweep launches of the plan on its stream with HIP events (the
+// kernels' own durations plus the gaps between back-to-back launches).
+cw_status time_launches(Plan& pl, long n, double* ms) {
+    hipEvent_t e0, e1;
+    CW_HIP(hipEventCreate(&e0));
+    CW_HIP(hipEventCreate(&e1));
+    cw_status st = CW_OK;
+    run_reset(pl);
+    CW_HIP(hipEventRecord(e0, pl.stream));
+    for (long i = 0; i < n && st == CW_OK; i++)
+        st = one_pass(pl, pl.three_d ? 1 : pl.s_main, CRIT_NONE, 0, 0.0, 0x7fffffff);
+    CW_HIP(hipEventRecord(e1, pl.stream));
+    CW_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    CW_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) return st;
+    *ms = (double)t / (double)n;
+    return CW_OK;
+}
 
 cw_status check_opts(const cw_relax_opts* o, Opts* out) {
     if (!o) return fail(CW_ERR_ARG, "null options");
@@ -506,22 +528,25 @@ cw_status check_opts(const cw_relax_opts* o, Opts* out) {
     out->precision = o->precision;
     out->s = o->sweeps_per_pass;
     out->check_every = o->check_every > 0 ? o->check_every : 0;
-    out->inner_tol = o->inner_tolerance > 0.0 ? o->inner_tolerance : 1e-4;
-    out->max_outer = o->max_outer > 0 ? o->max_outer : 40;
+    out->inner_tol = o->inner_tolerance > 0.0 ? o->inner_tolerance : 1e-3;
+    out->max_outer = o->max_outer > 0 ? o->max_outer : 1000;
     return CW_OK;
 }
 
+uint64_t span_voxels(const cw_field* f) {
+    return (uint64_t)(f->span[1] - f->span[0] + 1) * (f->span[3] - f->span[2] + 1) *
+           (f->span[5] - f->span[4] + 1);
+}
+
 // Plain FP64 solve of a set of fields (each sweeps its own span box).
-cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_stats* stats,
-                     bool* all_ok) {
+cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_stats* stats) {
     cw_status st;
     hipStream_t stream = F[0]->stream;
     const bool three_d = F[0]->D > 1;
     for (uint32_t i = 0; i < nf; i++)
         if ((F[i]->D > 1) != three_d)
             return fail(CW_ERR_ARG, "a batch mixes planes and volumes");
-    int s = o.s ? o.s : 2;
-    if (s > 2) s = 2;
+    int s = o.s ? std::min(o.s, 2) : 2;
     if (three_d) s = 1;
     Plan pl;
     pl.stream = stream;
@@ -532,18 +557,19 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     std::vector<double> omegas(nf);
     for (uint32_t i = 0; i < nf; i++) {
         cw_field* f = F[i];
-        if (stats) {
-            memset(&stats[i], 0, sizeof(cw_relax_stats));
-            stats[i].rel_residual = std::nan("");
-            stats[i].sweeps_per_pass = s;
-        }
+        memset(&stats[i], 0, sizeof(cw_relax_stats));
+        stats[i].rel_residual = std::nan("");
+        stats[i].sweeps_per_pass = s;
         if (!f->has_span) {
-            if (stats) { stats[i].sweeps = o.crit == CRIT_MAXCHANGE ? 1 : 0; stats[i].status = CW_OK; }
+            // Nothing to balance: Ok(1) (pde.rs:256-259).
+            stats[i].sweeps = o.crit == CRIT_MAXCHANGE ? 1 : 0;
             continue;
         }
         omegas[i] = o.omega > 0.0 ? o.omega : reference_omega(f);
-        if (o.max_sweeps == 0) {
-            if (stats) stats[i].status = CW_ERR_NOT_CONVERGED;
+        stats[i].omega = omegas[i];
+        stats[i].span_voxels = span_voxels(f);
+        if (o.max_sweeps == 0 && o.time_passes == 0) {
+            stats[i].status = CW_ERR_NOT_CONVERGED;
             continue;
         }
         // Outside the swept box both buffers must agree (pde.rs:249-252, :278).
@@ -553,7 +579,6 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
         fill_prob_common(pr, f, omegas[i]);
         pr.x[0] = f->c[f->cur];
         pr.x[1] = three_d ? f->c[f->cur] : f->c[f->cur ^ 1];
-        pr.in_place = three_d;
         pr.k = f->k;
         pr.k2 = f->general ? f->k2 : nullptr;
         pr.c64 = f->c[f->cur];
@@ -566,76 +591,91 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     for (uint32_t i = 0; i < nf; i++)
         if (F[i]->stream != stream) CW_HIP(hipStreamSynchronize(F[i]->stream));
     pl.nprob = (int)idx.size();
-    bool ok = true;
-    if (pl.nprob > 0) {
-        if ((st = plan_upload(pl))) return st;
-        if ((st = plan_set_tiling(pl, TL_MAIN))) return st;
-        int check_every = o.check_every ? o.check_every : (three_d ? 8 : 1);
-        LoopResult lr;
-        if (o.crit == CRIT_RESID) {
+    if (pl.nprob == 0) return CW_OK;
+    if ((st = plan_upload(pl))) return st;
+    if (o.time_passes > 0) {
+        if ((st = time_launches(pl, o.time_passes, o.time_ms))) return st;
+        for (int p = 0; p < pl.nprob; p++)
+            if (!three_d && (pl.pass & 1)) F[idx[p]]->cur ^= 1;
+        return CW_OK;
+    }
+    const int check_every = o.check_every ? o.check_every : (three_d ? 8 : 4);
+    LoopResult lr;
+    run_reset(pl);
+    if (o.crit == CRIT_RESID) {
+        if ((st = run_resid(pl, CRIT_RESID, o.tol, 0))) return st;
+    }
+    if (three_d && o.crit == CRIT_RESID) {
+        // Volumes test the residual in its own pass every check_every sweeps.
+        while (any_status(pl, ST_RUNNING) && (uint64_t)pl.sweeps_before < o.max_sweeps) {
+            const uint64_t n =
+                std::min<uint64_t>(check_every, o.max_sweeps - (uint64_t)pl.sweeps_before);
+            if ((st = run_passes(pl, CRIT_NONE, o.tol, n, o.max_sweeps, 1, &lr))) return st;
             if ((st = run_resid(pl, CRIT_RESID, o.tol, 0))) return st;
-            if ((st = read_ctl(pl))) return st;
         }
-        if (three_d && o.crit == CRIT_RESID) {
-            // Volumes test the residual in its own pass every check_every sweeps.
-            uint64_t done = 0;
-            while (any_status(pl, ST_RUNNING) && done < o.max_sweeps) {
-                const uint64_t n = std::min<uint64_t>(check_every, o.max_sweeps - done);
-                if ((st = run_passes(pl, true, CRIT_NONE, o.tol, n, done + n, 1, &lr))) return st;
-                // run_passes stops problems at the running max; re-arm the ones maxed early.
-                for (int p = 0; p < pl.nprob; p++)
-                    if (pl.hc[p].status == ST_MAXED && (uint64_t)pl.hc[p].sweeps < o.max_sweeps &&
-                        !pl.hc[p].nonfinite)
-                        pl.hc[p].status = ST_RUNNING;
-                if ((st = write_ctl(pl))) return st;
-                done += n;
-                if ((st = run_resid(pl, CRIT_RESID, o.tol, 0))) return st;
-                if ((st = read_ctl(pl))) return st;
-            }
-        } else {
-            if ((st = run_passes(pl, true, o.crit, o.tol, o.max_sweeps, o.max_sweeps, check_every,
-                                 &lr)))
-                return st;
-        }
-        for (int p = 0; p < pl.nprob; p++) {
-            cw_field* f = F[idx[p]];
-            const Ctl& c = pl.hc[p];
-            if (!three_d && (c.pass & 1)) f->cur ^= 1;
-            cw_status fs = CW_OK;
-            if (c.nonfinite) fs = CW_ERR_NONFINITE;
-            else if (c.status != ST_DONE) fs = CW_ERR_NOT_CONVERGED;
-            if (fs != CW_OK) ok = false;
-            if (stats) {
-                cw_relax_stats& S = stats[idx[p]];
-                S.sweeps = (uint64_t)c.sweeps;
-                S.passes = (uint64_t)lr.passes;
-                S.status = fs;
-                S.omega = omegas[idx[p]];
-                S.span_voxels = (uint64_t)(f->span[1] - f->span[0] + 1) *
-                                (f->span[3] - f->span[2] + 1) * (f->span[5] - f->span[4] + 1);
-                if (o.crit == CRIT_RESID && c.b2 > 0.0) S.rel_residual = std::sqrt(c.r2 / c.b2);
-            }
-        }
+        if ((st = launch_decide(pl, DECIDE_PASS, CRIT_NONE, o.tol,
+                                (int)std::min<uint64_t>(o.max_sweeps, 0x7fffffff))))
+            return st;
+        if ((st = read_ctl(pl))) return st;
+    } else {
+        if ((st = run_passes(pl, o.crit, o.tol, o.max_sweeps, o.max_sweeps, check_every, &lr)))
+            return st;
     }
-    for (uint32_t i = 0; i < nf; i++) {
-        if (stats && stats[i].status != CW_OK) ok = false;
+    for (int p = 0; p < pl.nprob; p++) {
+        cw_field* f = F[idx[p]];
+        const Ctl& c = pl.hc[p];
+        if (!three_d && (c.parity & 1)) f->cur ^= 1;
+        cw_status fs = CW_OK;
+        if (c.nonfinite) fs = CW_ERR_NONFINITE;
+        else if (c.status != ST_DONE) fs = CW_ERR_NOT_CONVERGED;
+        cw_relax_stats& S = stats[idx[p]];
+        S.sweeps = (uint64_t)c.sweeps;
+        S.passes = (uint64_t)lr.passes;
+        S.status = fs;
+        if (o.crit == CRIT_RESID && c.b2 > 0.0) S.rel_residual = std::sqrt(c.r2 / c.b2);
     }
-    *all_ok = ok;
     return CW_OK;
 }
 
+// Sweep budget of the next FP32 segment of one problem. From the FP64
+// residuals at the last two corrections, the observed rate gives the sweeps
+// that reach the target, but one segment is asked for no more than
+// `per_segment` (FP32 cannot carry a correction much further); 10% margin,
+// clamped to [lo, hi]. While the residual has not yet started to fall (the
+// SOR transient) the segment doubles.
+int next_segment(const std::vector<double>& res, const std::vector<int>& at, double target,
+                 double per_segment, int first, int lo, int hi, int s) {
+    int k = first;
+    const size_t n = res.size();
+    if (n >= 2) {
+        const int last = at[n - 1] - at[n - 2];
+        if (res[n - 1] > 0.0 && res[n - 2] > res[n - 1] && last > 0) {
+            const double rate = std::log(res[n - 1] / res[n - 2]) / (double)last;
+            const double goal = std::max(target / res[n - 1], per_segment);
+            const double need = std::log(goal) / rate;
+            k = need > 0.0 ? (int)std::ceil(1.1 * need) + s : lo;
+        } else {
+            k = 2 * std::max(last, first);
+        }
+    }
+    k = std::max(lo, std::min(hi, k));
+    return (k + s - 1) / s * s;
+}
+
 // FP32 smoothing of the defect equation with FP64 residual and correction.
-//   r = b - A c (FP64), A e = r smoothed in FP32 with e >= -c (projected SOR),
+//   r = b - A c (FP64), A e = r smoothed in FP32 from e = 0 with e >= -c,
 //   c <- max(c + e, 0) (FP64), until ||rt|| / ||b_eff|| <= tol.
-cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_stats* stats,
-                       bool* all_ok) {
+// SOR is stationary, so restarting e from zero on the fresh FP64 residual
+// keeps the error the iteration works on: segments cost only their FP64
+// passes and keep the correction small enough for FP32.
+cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_stats* stats) {
     cw_status st;
     hipStream_t stream = F[0]->stream;
     const bool three_d = F[0]->D > 1;
     for (uint32_t i = 0; i < nf; i++)
         if ((F[i]->D > 1) != three_d)
             return fail(CW_ERR_ARG, "a batch mixes planes and volumes");
-    int s = o.s ? o.s : 2;
+    int s = o.s ? o.s : 4;
     if (three_d) s = 1;
     Plan outer, inner;
     outer.stream = inner.stream = stream;
@@ -646,16 +686,18 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     inner.s_main = s;
     std::vector<int> idx;
     std::vector<double> omegas(nf);
+    int longest = 2;
     for (uint32_t i = 0; i < nf; i++) {
         cw_field* f = F[i];
-        if (stats) {
-            memset(&stats[i], 0, sizeof(cw_relax_stats));
-            stats[i].rel_residual = std::nan("");
-            stats[i].sweeps_per_pass = s;
-        }
+        memset(&stats[i], 0, sizeof(cw_relax_stats));
+        stats[i].rel_residual = std::nan("");
+        stats[i].sweeps_per_pass = s;
         if (!f->has_span) continue;
         if ((st = ensure_fp32(f))) return st;
         omegas[i] = o.omega > 0.0 ? o.omega : reference_omega(f);
+        stats[i].omega = omegas[i];
+        stats[i].span_voxels = span_voxels(f);
+        for (int a = 0; a < 3; a++) longest = std::max(longest, f->span[2 * a + 1] - f->span[2 * a] + 1);
         Prob pr;
         fill_prob_common(pr, f, omegas[i]);
         pr.k = f->k;
@@ -670,7 +712,6 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         Prob ip = pr;
         ip.x[0] = f->e[0];
         ip.x[1] = three_d ? f->e[0] : f->e[1];
-        ip.in_place = three_d;
         ip.k = f->r32;
         ip.k2 = nullptr;
         ip.lb = pr.lb32;
@@ -682,87 +723,94 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         inner.hc.push_back(c);
         idx.push_back((int)i);
     }
-    outer.nprob = inner.nprob = (int)idx.size();
-    std::vector<uint64_t> inner_sweeps(nf, 0), outer_iters(nf, 0);
-    long passes = 0;
-    if (outer.nprob > 0) {
-        if ((st = plan_upload(outer))) return st;
-        if ((st = plan_set_tiling(outer, TL_MAIN))) return st;
-        if ((st = plan_upload(inner))) return st;
-        if ((st = plan_set_tiling(inner, TL_MAIN))) return st;
-        const int check_every = o.check_every ? o.check_every : (three_d ? 8 : 2);
-        if ((st = run_resid(outer, CRIT_RESID, o.tol, 1))) return st;
-        if ((st = read_ctl(outer))) return st;
-        uint64_t total_inner = 0;
-        for (uint64_t it = 0; it < o.max_outer && any_status(outer, ST_RUNNING); it++) {
-            const uint64_t left = o.max_sweeps > total_inner ? o.max_sweeps - total_inner : 0;
-            if (left == 0) break;
-            for (int p = 0; p < inner.nprob; p++) {
-                init_ctl(inner.hc[p], outer.hc[p].status == ST_RUNNING ? ST_RUNNING : ST_IDLE);
-                if (outer.hc[p].status == ST_RUNNING) outer_iters[idx[p]]++;
-            }
-            if ((st = write_ctl(inner))) return st;
-            LoopResult lr;
-            // Volumes smooth a fixed count per step, the asymptotic SOR rate
-            // rho = omega - 1 brought down by inner_tol (with 50% margin).
-            uint64_t budget = left;
-            if (three_d) {
-                double rho = std::max(0.5, omegas[idx[0]] - 1.0);
-                budget = std::min<uint64_t>(
-                    left, (uint64_t)(1.5 * std::log(o.inner_tol) / std::log(rho)) + 8);
-            }
-            if ((st = run_passes(inner, false, three_d ? CRIT_NONE : CRIT_INNER, o.inner_tol,
-                                 budget, budget, check_every, &lr)))
-                return st;
-            passes += lr.passes;
-            uint64_t most = 0;
-            for (int p = 0; p < inner.nprob; p++) {
-                if (outer.hc[p].status != ST_RUNNING) continue;
-                inner_sweeps[idx[p]] += (uint64_t)inner.hc[p].sweeps;
-                most = std::max<uint64_t>(most, (uint64_t)inner.hc[p].sweeps);
-            }
-            total_inner += most;
-            {
-                const int nt = (int)outer.tp[TL_EL].size();
-                PassArgs a = make_args(outer, TL_EL, MODE_PLAIN, CRIT_NONE, 0, 1, 0.0, 0);
-                a.inner_ctl = inner.dc;
-                hipLaunchKernelGGL(correct64, dim3(nt), dim3(256), 0, stream, a);
-                CW_HIP(hipGetLastError());
-            }
-            if ((st = run_resid(outer, CRIT_RESID, o.tol, 1))) return st;
-            if ((st = read_ctl(outer))) return st;
-        }
+    const int np = (int)idx.size();
+    outer.nprob = inner.nprob = np;
+    if (np == 0) return CW_OK;
+    if ((st = plan_upload(outer))) return st;
+    if ((st = plan_upload(inner))) return st;
+    run_reset(outer);
+    if ((st = run_resid(outer, CRIT_RESID, o.tol, 1))) return st;
+    if (o.time_passes > 0) {
+        for (int p = 0; p < np; p++) init_ctl(inner.hc[p], ST_RUNNING);
+        if ((st = write_ctl(inner))) return st;
+        return time_launches(inner, o.time_passes, o.time_ms);
     }
-    bool ok = true;
-    for (int p = 0; p < outer.nprob; p++) {
+    std::vector<std::vector<double>> hist(np);
+    std::vector<std::vector<int>> hist_at(np);
+    std::vector<int> total(np, 0), outer_it(np, 0);
+    for (int p = 0; p < np; p++) {
+        const Ctl& c = outer.hc[p];
+        hist[p].push_back(c.b2 > 0.0 ? std::sqrt(c.r2 / c.b2) : std::sqrt(c.r2));
+        hist_at[p].push_back(0);
+    }
+    // First segment: a fraction of the box's longest side, the scale of the
+    // SOR transient; later ones follow the measured rate.
+    const int first = std::max(32, longest / 4);
+    const int hi = std::max(256, 4 * longest);
+    long passes = 0;
+    for (uint64_t it = 0; it < o.max_outer && any_status(outer, ST_RUNNING); it++) {
+        int most = 0;
+        for (int p = 0; p < np; p++) {
+            const bool run = outer.hc[p].status == ST_RUNNING;
+            init_ctl(inner.hc[p], run ? ST_RUNNING : ST_IDLE);
+            if (!run) continue;
+            int k = next_segment(hist[p], hist_at[p], o.tol, o.inner_tol, first, 16, hi, s);
+            if ((uint64_t)total[p] + k > o.max_sweeps)
+                k = (int)(o.max_sweeps - (uint64_t)total[p]);
+            inner.hc[p].budget = std::max(k, 1);
+            most = std::max(most, inner.hc[p].budget);
+            outer_it[p]++;
+        }
+        if (most <= 0) break;
+        if ((st = write_ctl(inner))) return st;
+        run_reset(inner);
+        LoopResult lr;
+        if ((st = run_passes(inner, CRIT_NONE, 0.0, (uint64_t)most, 0x7fffffff, 1, &lr))) return st;
+        passes += lr.passes;
+        for (int p = 0; p < np; p++)
+            if (outer.hc[p].status == ST_RUNNING) total[p] += inner.hc[p].sweeps;
+        {
+            const int nt = (int)outer.tp[TL_EL].size();
+            PassArgs a = make_args(outer, TL_EL, MODE_SWEEP, CRIT_NONE, 0.0, 0);
+            a.inner_ctl = inner.dc;
+            hipLaunchKernelGGL(correct64, dim3(nt), dim3(256), 0, stream, a);
+            CW_HIP(hipGetLastError());
+        }
+        if ((st = run_resid(outer, CRIT_RESID, o.tol, 1))) return st;
+        bool any_left = false;
+        for (int p = 0; p < np; p++) {
+            const Ctl& c = outer.hc[p];
+            if (inner.hc[p].status == ST_IDLE) continue;
+            hist[p].push_back(c.b2 > 0.0 ? std::sqrt(c.r2 / c.b2) : std::sqrt(c.r2));
+            hist_at[p].push_back(total[p]);
+            if (c.status == ST_RUNNING && (uint64_t)total[p] < o.max_sweeps) any_left = true;
+        }
+        if (!any_left) break;
+    }
+    for (int p = 0; p < np; p++) {
         const Ctl& c = outer.hc[p];
         cw_status fs = CW_OK;
         if (c.nonfinite) fs = CW_ERR_NONFINITE;
         else if (c.status != ST_DONE) fs = CW_ERR_NOT_CONVERGED;
-        if (fs != CW_OK) ok = false;
-        if (stats) {
-            cw_relax_stats& S = stats[idx[p]];
-            cw_field* f = F[idx[p]];
-            S.sweeps = inner_sweeps[idx[p]];
-            S.passes = (uint64_t)passes;
-            S.outer_iters = outer_iters[idx[p]];
-            S.status = fs;
-            S.omega = omegas[idx[p]];
-            S.span_voxels = (uint64_t)(f->span[1] - f->span[0] + 1) *
-                            (f->span[3] - f->span[2] + 1) * (f->span[5] - f->span[4] + 1);
-            if (c.b2 > 0.0) S.rel_residual = std::sqrt(c.r2 / c.b2);
-        }
+        cw_relax_stats& S = stats[idx[p]];
+        S.sweeps = (uint64_t)total[p];
+        S.passes = (uint64_t)passes;
+        S.outer_iters = (uint64_t)outer_it[p];
+        S.status = fs;
+        if (c.b2 > 0.0) S.rel_residual = std::sqrt(c.r2 / c.b2);
     }
-    *all_ok = ok;
     return CW_OK;
 }
 
-cw_status solve(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts, cw_relax_stats* stats) {
+cw_status solve(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts, cw_relax_stats* stats,
+                long time_passes = 0, double* time_ms = nullptr) {
     auto t0 = std::chrono::steady_clock::now();
     if (!F || nf == 0) return fail(CW_ERR_ARG, "no fields");
     Opts o;
     cw_status st = check_opts(opts, &o);
     if (st) return st;
+    o.time_passes = time_passes;
+    o.time_ms = time_ms;
     for (uint32_t i = 0; i < nf; i++) {
         if (!F[i]) return fail(CW_ERR_ARG, "null field %u", i);
         if (F[i]->coef_dirty && (st = build_coefficients(F[i]))) return st;
@@ -772,11 +820,11 @@ cw_status solve(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts, cw_r
         local.resize(nf);
         stats = local.data();
     }
-    bool ok = true;
-    if (o.precision == CW_PREC_FP64) st = solve_fp64(F, nf, o, stats, &ok);
-    else st = solve_refine(F, nf, o, stats, &ok);
+    if (o.precision == CW_PREC_FP64) st = solve_fp64(F, nf, o, stats);
+    else st = solve_refine(F, nf, o, stats);
     if (st) return st;
     CW_HIP(hipStreamSynchronize(F[0]->stream));
+    if (time_passes > 0) return CW_OK;
     const double secs =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     cw_status worst = CW_OK;
@@ -784,15 +832,11 @@ cw_status solve(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts, cw_r
         stats[i].seconds = secs;
         if (stats[i].status != CW_OK && worst == CW_OK) worst = (cw_status)stats[i].status;
     }
-    if (worst == CW_ERR_NOT_CONVERGED) {
-        for (uint32_t i = 0; i < nf; i++)
-            if (stats[i].status == CW_ERR_NOT_CONVERGED)
-                return fail(CW_ERR_NOT_CONVERGED,
-                            "diffusion did not settle within %llu sweeps at tolerance %g",
-                            (unsigned long long)o.max_sweeps, o.tol);
-    }
+    if (worst == CW_ERR_NOT_CONVERGED)
+        return fail(CW_ERR_NOT_CONVERGED,
+                    "diffusion did not settle within %llu sweeps at tolerance %g",
+                    (unsigned long long)o.max_sweeps, o.tol);
     if (worst == CW_ERR_NONFINITE) return fail(CW_ERR_NONFINITE, "the residual is not finite");
-    (void)ok;
     return CW_OK;
 }
 
@@ -943,6 +987,24 @@ cw_status cw_batch_relax(cw_field* const* fields, uint32_t count, const cw_relax
     return solve(fields, count, opts, stats);
 }
 
+cw_status cw_batch_time_passes(cw_field* const* fields, uint32_t count, const cw_relax_opts* opts,
+                               uint32_t passes, double* ms_per_pass) {
+    if (!fields || count == 0 || !opts || !ms_per_pass || passes == 0)
+        return fail(CW_ERR_ARG, "bad argument");
+    return solve(fields, count, opts, nullptr, (long)passes, ms_per_pass);
+}
+
+cw_status cw_field_fill(cw_field* f, double value) {
+    if (!f) return fail(CW_ERR_ARG, "null field");
+    const size_t rows = (size_t)f->H * f->D;
+    const int blocks = (int)std::min<size_t>((rows * f->W + 255) / 256, 65536);
+    hipLaunchKernelGGL(fill_cells, dim3(blocks), dim3(256), 0, f->stream, f->c[f->cur],
+                       (int)f->W, (long)rows, f->P, value);
+    CW_HIP(hipGetLastError());
+    CW_HIP(hipStreamSynchronize(f->stream));
+    return CW_OK;
+}
+
 cw_status cw_field_residual(cw_field* f, double* rel) {
     if (!f || !rel) return fail(CW_ERR_ARG, "null argument");
     cw_status st;
@@ -964,9 +1026,8 @@ cw_status cw_field_residual(cw_field* f, double* rel) {
     pl.hc.push_back(c);
     pl.nprob = 1;
     if ((st = plan_upload(pl))) return st;
-    if ((st = plan_set_tiling(pl, TL_MAIN))) return st;
+    run_reset(pl);
     if ((st = run_resid(pl, CRIT_NONE, 0.0, 0))) return st;
-    if ((st = read_ctl(pl))) return st;
     const Ctl& r = pl.hc[0];
     *rel = r.b2 > 0.0 ? std::sqrt(r.r2 / r.b2) : std::sqrt(r.r2);
     return CW_OK;

@@ -111,6 +111,10 @@ class Field:
             hp = hb.ctypes.data_as(C.POINTER(C.c_uint8))
         _check(_abi.lib().cw_field_upload(self._h, dbl(values), hp, dbl(uptake), dbl(production)))
 
+    def fill(self, value: float):
+        """ScalarField::fill (pde.rs:212-215), on the device."""
+        _check(_abi.lib().cw_field_fill(self._h, value))
+
     def download(self) -> np.ndarray:
         out = np.empty(self.n, dtype=np.float64)
         _check(_abi.lib().cw_field_download(self._h, _dptr(out)))
@@ -146,6 +150,17 @@ def batch_relax(fields, check=True, **kw):
     return [s.as_dict() for s in stats], rc
 
 
-__all__ = ["Field", "batch_relax", "make_opts", "CellweaveError", "ConfigError",
+def time_passes(fields, passes, **kw) -> float:
+    """Average milliseconds of one sweep-kernel launch of the plan
+    cw_batch_relax would build (HIP events on the library stream)."""
+    o = make_opts(**kw)
+    n = len(fields)
+    arr = (C.c_void_p * n)(*[f._h.value for f in fields])
+    ms = C.c_double()
+    _check(_abi.lib().cw_batch_time_passes(arr, n, C.byref(o), passes, C.byref(ms)))
+    return ms.value
+
+
+__all__ = ["time_passes", "Field", "batch_relax", "make_opts", "CellweaveError", "ConfigError",
            "SimulationError", "DeviceError", "CRIT_MAX_CHANGE", "CRIT_REL_RESIDUAL",
            "PREC_FP64", "PREC_FP32_REFINE"]

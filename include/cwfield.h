@@ -109,6 +109,10 @@ cw_status cw_field_set_source(cw_field *field, double source);
 cw_status cw_field_upload(cw_field *field, const double *values, const uint8_t *held,
                           const double *uptake, const double *production);
 
+/* ScalarField::fill (pde.rs:212-215): every voxel of the device values set
+ * to `value`, held ones included, with no host traffic. */
+cw_status cw_field_fill(cw_field *field, double value);
+
 /* Copy the device values back to the host (n doubles). */
 cw_status cw_field_download(const cw_field *field, double *values);
 
@@ -132,6 +136,14 @@ cw_status cw_field_residual(cw_field *field, double *rel_residual);
  * Returns CW_ERR_NOT_CONVERGED if any field did not converge. */
 cw_status cw_batch_relax(cw_field *const *fields, uint32_t count, const cw_relax_opts *opts,
                          cw_relax_stats *stats);
+
+/* Measurement helper (no reference counterpart): builds the same launch plan
+ * cw_batch_relax would for `opts` and times `passes` back-to-back launches of
+ * its sweep kernel with HIP events on the library's stream. *ms_per_pass gets
+ * the average. The fields' values are swept (not solved) by the call. */
+cw_status cw_batch_time_passes(cw_field *const *fields, uint32_t count,
+                               const cw_relax_opts *opts, uint32_t passes,
+                               double *ms_per_pass);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,187 @@
+"""GPU parity: the HIP solve against the CPU oracle through the C ABI.
+
+The plain FP64 solve must equal the oracle bit for bit (values and sweep
+count) on the reference's own fixtures (pde.rs tests) and on the synthetic
+workloads; the mixed-precision solve must reach the FP64 residual target.
+"""
+import numpy as np
+import pytest
+
+from cellweave_amd import cases
+from cellweave_amd.field import (CRIT_MAX_CHANGE, CRIT_REL_RESIDUAL, PREC_FP32_REFINE, PREC_FP64,
+                                 Field, SimulationError, batch_relax)
+from helpers import (D, bathed_disc_inputs, bits_equal, gpu_from_case, gpu_from_oracle,
+                     oracle_from_case, outside_disc)
+from oracle.orf import OracleField, RelaxError
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("size,radius,uptake,tol", [
+    (30, 6, 0.004, 1e-11),     # pde.rs:783-807
+    (32, 8, 0.003, 1e-11),     # pde.rs:810-830
+    (61, 20, 0.0005, 1e-11),   # pde.rs:982-1000, odd box
+    (60, 6, 0.004, 1e-11),
+])
+def test_bathed_disc_bitwise(size, radius, uptake, tol):
+    say(f"case bathed_disc size={size} radius={radius} uptake={uptake}")
+    ref = bathed_disc_inputs(size, radius, uptake)
+    g = gpu_from_oracle(ref)
+    want = ref.relax(tol, 50_000)
+    got = g.relax(tol, 50_000)
+    assert got == want
+    assert bits_equal(g.download(), ref.values)
+
+
+@pytest.mark.parametrize("spp", [1, 2])
+def test_config1_max_change_bitwise(spp):
+    say(f"case config1 omega=1.9 spp={spp}")
+    (fc,) = cases.config1()
+    ref = oracle_from_case(fc)
+    g = gpu_from_case(fc)
+    want = ref.relax(1e-10, 100_000, omega=1.9)
+    st = g.relax_ex(omega=1.9, criterion=CRIT_MAX_CHANGE, tolerance=1e-10, max_sweeps=100_000,
+                    sweeps_per_pass=spp)
+    assert st["sweeps"] == want
+    assert bits_equal(g.download(), ref.values)
+
+
+def test_config1_reference_dropin():
+    say("case config1 reference relax (omega rule 1.95)")
+    (fc,) = cases.config1()
+    ref = oracle_from_case(fc)
+    g = gpu_from_case(fc)
+    assert g.relax(1e-9, 100_000) == ref.relax(1e-9, 100_000)
+    assert bits_equal(g.download(), ref.values)
+
+
+def test_config1_rel_residual_bitwise():
+    say("case config1 rel residual 1e-8 spp=1 check every sweep")
+    (fc,) = cases.config1()
+    ref = oracle_from_case(fc)
+    g = gpu_from_case(fc)
+    want = ref.relax_residual(1e-8, 100_000, omega=1.9)
+    st = g.relax_ex(omega=1.9, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, max_sweeps=100_000,
+                    sweeps_per_pass=1, check_every=1)
+    assert st["sweeps"] == want
+    assert bits_equal(g.download(), ref.values)
+    rel = g.residual()
+    assert rel <= 1e-8
+    assert abs(rel - ref.rel_residual()[0]) <= 1e-12 * max(1.0, rel) + 1e-15
+
+
+def test_general_form_bitwise():
+    """A uniform source and voxels that both take up and secrete: the
+    unfolded ((d*sum + source) + p) - u form."""
+    say("case general form 40x40 source")
+    n = 40
+    rng = np.random.default_rng(7)
+    ref = OracleField(n, n, 1, 0.3, 0.01, 0.5)
+    ref.set_source(2e-3)
+    ref.uptake[:] = rng.uniform(0, 4e-3, n * n)
+    ref.production[:] = np.where(rng.random(n * n) < 0.5, rng.uniform(0, 2e-3, n * n), 0.0)
+    ring = np.zeros((n, n), np.uint8)
+    ring[0, :] = ring[-1, :] = ring[:, 0] = ring[:, -1] = 1
+    ref.hold_where(0.7, ring.reshape(-1))
+    ref.copy_next_from_values()
+    g = Field(n, n, 1, 0.3, 0.01, 0.5)
+    g.set_source(2e-3)
+    g.upload(values=ref.values.copy(), held=ref.held.copy(), uptake=ref.uptake.copy(),
+             production=ref.production.copy())
+    want = ref.relax(1e-12, 100_000)
+    assert g.relax(1e-12, 100_000) == want
+    assert bits_equal(g.download(), ref.values)
+
+
+def test_starving_ball_3d_bitwise():
+    """pde.rs:1203-1229: a ball taking up in a bath, in a volume."""
+    say("case 3d ball 31^3")
+    side, r, u = 31, 10.0, 0.006
+    c = side // 2
+    zz, yy, xx = np.meshgrid(np.arange(side), np.arange(side), np.arange(side), indexing="ij")
+    inside = ((xx - c) ** 2 + (yy - c) ** 2 + (zz - c) ** 2 <= r * r).reshape(-1)
+    ref = OracleField(side, side, side, D, 0.0, 1.0)
+    ref.uptake[:] = np.where(inside, u, 0.0)
+    ref.hold_bath_at(1.0, (~inside).astype(np.uint8))
+    ref.copy_next_from_values()
+    g = gpu_from_oracle(ref)
+    want = ref.relax(1e-10, 1_000_000)
+    assert g.relax(1e-10, 1_000_000) == want
+    assert bits_equal(g.download(), ref.values)
+    centre = ref.value_at((side // 2) * (side * side + side + 1))
+    assert abs(centre - (1.0 - u * r * r / (6.0 * D))) < 0.05
+
+
+def test_max_sweeps_is_an_error():
+    """pde.rs:1030-1045."""
+    say("case max_sweeps 3")
+    ref = bathed_disc_inputs(30, 6, 0.004)
+    g = gpu_from_oracle(ref)
+    with pytest.raises(SimulationError, match="did not settle within 3 sweeps"):
+        g.relax(1e-9, 3)
+    with pytest.raises(RelaxError):
+        ref.relax(1e-9, 3)
+    # the first three sweeps still ran, as the reference's loop runs them
+    ref2 = bathed_disc_inputs(30, 6, 0.004)
+    with pytest.raises(RelaxError):
+        ref2.relax(1e-9, 3)
+    assert bits_equal(g.download(), ref2.values)
+
+
+def test_nothing_free_takes_one_sweep():
+    """pde.rs:1159-1175: all held, relax is Ok(1)."""
+    say("case nothing free")
+    g = Field(20, 20, 1, D, 0.0, 0.0)
+    g.upload(held=np.ones(400, np.uint8), values=np.ones(400))
+    assert g.relax(1e-9, 10) == 1
+
+
+def test_settled_field_takes_one_sweep():
+    """pde.rs:1015-1027."""
+    say("case settled re-relax")
+    ref = bathed_disc_inputs(30, 6, 0.004)
+    g = gpu_from_oracle(ref)
+    g.relax(1e-11, 50_000)
+    settled = g.download()
+    assert g.relax(1e-11, 50_000) == 1
+    assert np.max(np.abs(g.download() - settled)) < 1e-6
+
+
+def test_batch_equals_single():
+    say("case batch of three fields 256^2")
+    fields = cases.config2(n=256)
+    singles, batched = [], []
+    for fc in fields:
+        g = gpu_from_case(fc)
+        st = g.relax_ex(criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, max_sweeps=200_000,
+                        omega=1.97)
+        singles.append((st["sweeps"], g.download()))
+    gs = [gpu_from_case(fc) for fc in fields]
+    stats, _ = batch_relax(gs, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, max_sweeps=200_000,
+                           omega=1.97)
+    for (sw, v), st, g in zip(singles, stats, gs):
+        assert st["sweeps"] == sw
+        assert bits_equal(g.download(), v)
+
+
+def test_fp32_refine_reaches_residual_config1():
+    say("case config1 fp32 refine")
+    (fc,) = cases.config1()
+    ref = oracle_from_case(fc)
+    ref.relax_residual(1e-12, 200_000, omega=1.9)
+    g = gpu_from_case(fc)
+    st = g.relax_ex(omega=1.9, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, max_sweeps=200_000,
+                    precision=PREC_FP32_REFINE)
+    say(f"  refine stats {st}")
+    v = g.download()
+    chk = oracle_from_case(fc)
+    chk.values[:] = v
+    rel = chk.rel_residual()[0]
+    assert rel <= 1e-8, rel
+    err = np.max(np.abs(v - ref.values))
+    assert err < 1e-6, err
+    assert st["outer_iters"] >= 1
