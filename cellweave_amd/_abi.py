@@ -30,6 +30,7 @@ class RelaxOpts(C.Structure):
         ("check_every", C.c_int32),
         ("inner_tolerance", C.c_double),
         ("max_outer", C.c_uint64),
+        ("tile_shape", C.c_int32),
     ]
 
 

@@ -272,40 +272,6 @@ __device__ __forceinline__ int real_count(int gx, int gy, int gz, int W, int H, 
     return (gx > 0) + (gx < W - 1) + (gy > 0) + (gy < H - 1) + (gz > 0) + (gz < D - 1);
 }
 
-template <typename T> struct Arith;
-template <> struct Arith<double> {
-    // Reference order, no contraction (file compiled -ffp-contract=off).
-    static __device__ __forceinline__ double balanced(double d, double sum, double k, double k2,
-                                                       double source, bool general, int real,
-                                                       double decay) {
-        double num = d * sum;
-        if (general) {
-            num = num + source;
-            num = num + k;
-            num = num - k2;
-        } else {
-            num = num + k;
-        }
-        return num / ((double)real * d + decay);
-    }
-    static __device__ __forceinline__ double relax(double omo, double omega, double before,
-                                                    double bal) {
-        return omo * before + omega * bal;
-    }
-};
-template <> struct Arith<float> {
-    // A smoother only: the fast reciprocal (1 ulp) replaces the division.
-    static __device__ __forceinline__ float balanced(float d, float sum, float k, float, float,
-                                                     bool, int real, float decay) {
-        return __builtin_fmaf(d, sum, k) *
-               __builtin_amdgcn_rcpf(__builtin_fmaf((float)real, d, decay));
-    }
-    static __device__ __forceinline__ float relax(float omo, float omega, float before,
-                                                  float bal) {
-        return __builtin_fmaf(omo, before, omega * bal);
-    }
-};
-
 template <typename T> __device__ __forceinline__ T nan_of();
 template <> __device__ __forceinline__ double nan_of<double>() {
     return __longlong_as_double(0x7ff8000000000000ll);
@@ -313,7 +279,7 @@ template <> __device__ __forceinline__ double nan_of<double>() {
 template <> __device__ __forceinline__ float nan_of<float>() { return __int_as_float(0x7fc00000); }
 
 // ---------------------------------------------------------------------------
-// 2D LDS tile sweep: s full sweeps (2s half sweeps) per launch.
+// 2D LDS tile sweeps: s full sweeps (2s half sweeps) per launch.
 //
 // A workgroup loads a region 128 wide and RY tall (halo H = 2s around a
 // (128-2H) x (RY-2H) interior) from src into LDS, split by the parity of the
@@ -323,19 +289,30 @@ template <> __device__ __forceinline__ float nan_of<float>() { return __int_as_f
 // stored by colour so every register index is a compile-time constant. Half
 // sweep h updates the cells of colour h&1 at distance > h from the region's
 // edge (the rest is stale halo), so after 2s half sweeps the interior is exact
-// and is stored to dst.
+// and is stored to dst. The state loads are issued before the stop verdict
+// (the input parity of a sweep launch is known from the pass index), so their
+// latency overlaps it.
 // ---------------------------------------------------------------------------
-template <typename T, int S, int RY, int NW, bool PLAIN>
-__global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a) {
-    constexpr int H = 2 * S;
-    constexpr int RX = 128;
-    constexpr int TX = RX - 2 * H;
-    constexpr int TY = RY - 2 * H;
-    constexpr int NR = RY / NW;
+template <int RY, int NW, int S>
+struct Region {
+    static constexpr int H = 2 * S;
+    static constexpr int RX = 128;
+    static constexpr int TX = RX - 2 * H;
+    static constexpr int TY = RY - 2 * H;
+    static constexpr int NR = RY / NW;
     static_assert(RY % NW == 0 && NW % 2 == 0, "rows split evenly over an even wave count");
     static_assert(TY > 0, "interior rows");
-    using T2 = typename V2<T>::type;
-    __shared__ T cs[RY][2][64];
+};
+
+// FP64 plain sweep: the reference's arithmetic and floor, the per-sweep
+// largest change for the reference stopping rule, and optionally the
+// residual of the input for the relative-residual rule.
+template <int S, int RY, int NW, bool GEN>
+__global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a) {
+    using R = Region<RY, NW, S>;
+    constexpr int H = R::H, RX = R::RX, TX = R::TX, TY = R::TY, NR = R::NR;
+    static_assert(S <= kMaxS, "change slots");
+    __shared__ double cs[RY][2][64];
 
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
@@ -347,58 +324,58 @@ __global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a) {
     const int ti = lt % pr.ntx[tl], tj = lt / pr.ntx[tl];
     const int gxo = pr.xs + ti * TX - H;
     const int gyo = pr.ys + tj * TY - H;
-    const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
-    const int gz = 0;
+    const int W = pr.W, Hh = pr.H, P = pr.P;
     const int gx = gxo + 2 * lane;
     const bool colin = gx >= 0 && gx < W;
-    const T* kk = (const T*)pr.k;
-    const T* kk2 = (const T*)pr.k2;
-    const T* lbp = (const T*)pr.lb;
-    const bool general = PLAIN && kk2 != nullptr;
-    const bool has_lb = !PLAIN && lbp != nullptr;
+    const double* kk = (const double*)pr.k;
+    const double* kk2 = (const double*)pr.k2;
+    constexpr bool general = GEN;  // separate production and uptake (source != 0 or both set)
+    // real neighbours: x part per cell of the pair, y part per row (2D: D = 1)
+    const int rxe = (gx > 0) + (gx < W - 1), rxo = (gx + 1 > 0) + (gx + 1 < W - 1);
+    const int base = (gxo + gyo) & 1;
+    const int cb = (base + wv) & 1;  // colour of the even column on this wave's rows
+    const int rx0 = cb ? rxo : rxe, rx1 = cb ? rxe : rxo;  // by colour
 
-    // Coefficients of the lane's pair, by colour: cb is the colour of the even
-    // column (the same for every row of this wave).
-    const int base = (gxo + gyo + gz) & 1;
-    const int cb = (base + wv) & 1;
-    T k0[NR], k1[NR], u0[NR], u1[NR], l0[NR], l1[NR];
+    double k0[NR], k1[NR], u0[GEN ? NR : 1], u1[GEN ? NR : 1], v0[NR], v1[NR];
+    const bool replay = a.mode == MODE_REPLAY;
 #pragma unroll
     for (int i = 0; i < NR; i++) {
         const int gy = gyo + wv + NW * i;
-        T ce = nan_of<T>(), co = nan_of<T>(), ue = T(0), uo = T(0), le = T(0), lo = T(0);
+        double ce = nan_of<double>(), co = nan_of<double>(), ue = 0.0, uo = 0.0;
+        v0[i] = 0.0; v1[i] = 0.0;
         if (colin && gy >= 0 && gy < Hh) {
             const size_t off = (size_t)gy * P + gx;
-            const T2 c = *(const T2*)(kk + off);
+            const double2 c = *(const double2*)(kk + off);
             ce = c.x; co = c.y;
-            if (general) { const T2 u = *(const T2*)(kk2 + off); ue = u.x; uo = u.y; }
-            if (has_lb) { const T2 l = *(const T2*)(lbp + off); le = l.x; lo = l.y; }
+            if (general) { const double2 u = *(const double2*)(kk2 + off); ue = u.x; uo = u.y; }
+            if (!replay) {
+                const double2 v = *(const double2*)((const double*)pr.x[a.pass & 1] + off);
+                v0[i] = v.x; v1[i] = v.y;
+            }
         }
         k0[i] = cb ? co : ce; k1[i] = cb ? ce : co;
-        u0[i] = cb ? uo : ue; u1[i] = cb ? ue : uo;
-        l0[i] = cb ? lo : le; l1[i] = cb ? le : lo;
+        if (GEN) { u0[GEN ? i : 0] = cb ? uo : ue; u1[GEN ? i : 0] = cb ? ue : uo; }
     }
 
     int par;
     if (!sweep_prologue(a, pr, p, lt, &par)) return;
-    const T* src = (const T*)pr.x[par];
-    T* dst = (T*)pr.x[par ^ 1];
+    const double* src = (const double*)pr.x[par];
+    double* dst = (double*)pr.x[par ^ 1];
 
 #pragma unroll
     for (int i = 0; i < NR; i++) {
         const int y = wv + NW * i;
         const int gy = gyo + y;
-        T v0 = T(0), v1 = T(0);
-        if (colin && gy >= 0 && gy < Hh) {
-            const T2 v = *(const T2*)(src + (size_t)gy * P + gx);
-            v0 = v.x; v1 = v.y;
+        if (replay && colin && gy >= 0 && gy < Hh) {
+            const double2 v = *(const double2*)(src + (size_t)gy * P + gx);
+            v0[i] = v.x; v1[i] = v.y;
         }
-        cs[y][0][lane] = v0;
-        cs[y][1][lane] = v1;
+        cs[y][0][lane] = v0[i];
+        cs[y][1][lane] = v1[i];
     }
     __syncthreads();
 
-    const T d = (T)pr.d, decay = (T)pr.decay, omega = (T)pr.omega, omo = (T)pr.omo;
-    const T source = (T)pr.source;
+    const double d = pr.d, decay = pr.decay, omega = pr.omega, omo = pr.omo, source = pr.source;
 
     // Residual of the input state on the interior (projected natural residual).
     double r2 = 0.0;
@@ -407,30 +384,24 @@ __global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a) {
         for (int i = 0; i < NR; i++) {
             const int y = wv + NW * i;
             if (y < H || y >= H + TY) continue;
+            const int gy = gyo + y;
+            const int ry = (gy > 0) + (gy < Hh - 1);
 #pragma unroll
             for (int col = 0; col < 2; col++) {
                 const int q = col ^ cb;
                 const int x = 2 * lane + q;
                 if (x < H || x >= H + TX) continue;
-                const T kq = col ? k1[i] : k0[i];
+                const double kq = col ? k1[i] : k0[i];
                 if (kq != kq) continue;
-                const T c = cs[y][q][lane];
-                const T w = q ? cs[y][0][lane] : cs[y][1][lane - 1];
-                const T e = q ? cs[y][0][lane + 1] : cs[y][1][lane];
-                T sum = T(0);
+                const double c = cs[y][q][lane];
+                const double w = q ? cs[y][0][lane] : cs[y][1][lane - 1];
+                const double e = q ? cs[y][0][lane + 1] : cs[y][1][lane];
+                double sum = 0.0;
                 sum += w; sum += e; sum += cs[y - 1][q][lane]; sum += cs[y + 1][q][lane];
-                const int real = real_count(gxo + x, gyo + y, gz, W, Hh, D);
-                const double A = (double)real * (double)d + (double)decay;
-                double b;
-                if (general) b = ((double)source + (double)kq) - (double)(col ? u1[i] : u0[i]);
-                else b = (double)kq;
-                double r = (b + (double)d * (double)sum) - A * (double)c;
-                if (PLAIN) {
-                    if (A > 0.0 && !((double)c + r / A > 0.0)) r = -A * (double)c;
-                } else if (has_lb) {
-                    const double lo = (double)(col ? l1[i] : l0[i]);
-                    if (A > 0.0 && !((double)c + r / A > lo)) r = A * (lo - (double)c);
-                }
+                const double A = (double)((col ? rx1 : rx0) + ry) * d + decay;
+                const double b = general ? ((source + kq) - (col ? u1[GEN ? i : 0] : u0[GEN ? i : 0])) : kq;
+                double r = (b + d * sum) - A * c;
+                if (A > 0.0 && !(c + r / A > 0.0)) r = -A * c;
                 r2 += r * r;
             }
         }
@@ -444,32 +415,36 @@ __global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a) {
         const int colour = h & 1;
         const int lo = h + 1, hiY = RY - 2 - h, hiX = RX - 2 - h;
         const int q = colour ^ cb;
+        const int x = 2 * lane + q;
+        const bool xin = x >= lo && x <= hiX;
+        const int rx = colour ? rx1 : rx0;
 #pragma unroll
         for (int i = 0; i < NR; i++) {
             const int y = wv + NW * i;
             if (y < lo || y > hiY) continue;
-            const int x = 2 * lane + q;
-            if (x < lo || x > hiX) continue;
-            const T kq = colour ? k1[i] : k0[i];
-            if (kq != kq) continue;
-            const T before = cs[y][q][lane];
-            const T w = q ? cs[y][0][lane] : cs[y][1][lane - 1];
-            const T e = q ? cs[y][0][lane + 1] : cs[y][1][lane];
-            T sum = T(0);
+            const double kq = colour ? k1[i] : k0[i];
+            if (!xin || kq != kq) continue;
+            const int gy = gyo + y;
+            const int ry = (gy > 0) + (gy < Hh - 1);
+            const double before = cs[y][q][lane];
+            const double w = q ? cs[y][0][lane] : cs[y][1][lane - 1];
+            const double e = q ? cs[y][0][lane + 1] : cs[y][1][lane];
+            double sum = 0.0;
             sum += w; sum += e; sum += cs[y - 1][q][lane]; sum += cs[y + 1][q][lane];
-            const int real = real_count(gxo + x, gyo + y, gz, W, Hh, D);
-            const T bal = Arith<T>::balanced(d, sum, kq, colour ? u1[i] : u0[i], source, general,
-                                             real, decay);
-            T after = Arith<T>::relax(omo, omega, before, bal);
-            if (PLAIN) {
-                after = after > T(0) ? after : T(0);
-            } else if (has_lb) {
-                const T l = colour ? l1[i] : l0[i];
-                after = after > l ? after : l;
+            double num = d * sum;
+            if (general) {
+                num = num + source;
+                num = num + kq;
+                num = num - (colour ? u1[GEN ? i : 0] : u0[GEN ? i : 0]);
+            } else {
+                num = num + kq;
             }
+            const double bal = num / ((double)(rx + ry) * d + decay);
+            double after = omo * before + omega * bal;
+            after = after > 0.0 ? after : 0.0;
             cs[y][q][lane] = after;
-            if (PLAIN && y >= H && y < H + TY && x >= H && x < H + TX) {
-                const unsigned long long m = change_bits((double)__builtin_fabs(after - before));
+            if (y >= H && y < H + TY && x >= H && x < H + TX) {
+                const unsigned long long m = change_bits(__builtin_fabs(after - before));
                 mv[h >> 1] = m > mv[h >> 1] ? m : mv[h >> 1];
             }
         }
@@ -484,16 +459,125 @@ __global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a) {
             if (y < H || y >= H + TY) continue;
             const int gy = gyo + y;
             if (gy > pr.y1) continue;
+            double2 v;
+            v.x = cs[y][0][lane];
+            v.y = cs[y][1][lane];
+            *(double2*)(dst + (size_t)gy * P + gx) = v;
+        }
+    }
+
+    if (a.check) block_partial(a.partials + (a.pass & 1) * a.part_stride, t, r2, 0.0);
+    if (a.crit == CRIT_MAXCHANGE && a.mode == MODE_SWEEP) block_moved<NW, S>(a.ctl[p], a.pass, mv);
+}
+
+// Linear smoother in pre-scaled form: e <- max(lb, (1-omega) e + alpha sum + beta)
+// with alpha = omega d / A and beta = omega r / A (A = real d + decay), which
+// is the SOR update of A e = r written as two fused multiply-adds. beta is
+// the stored coefficient (NaN = held); lb, when present, bounds the FP32
+// correction from below so c + e stays >= 0 (the floor of the field).
+template <typename T, int S, int RY, int NW>
+__global__ __launch_bounds__(NW * 64) void smooth2d(PassArgs a) {
+    using R = Region<RY, NW, S>;
+    constexpr int H = R::H, RX = R::RX, TX = R::TX, TY = R::TY, NR = R::NR;
+    using T2 = typename V2<T>::type;
+    __shared__ T cs[RY][2][64];
+
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int p = a.tile_prob[t];
+    const Prob& pr = a.probs[p];
+    const int tl = a.tl;
+    const int lt = t - pr.sw_base[tl];
+    const int ti = lt % pr.ntx[tl], tj = lt / pr.ntx[tl];
+    const int gxo = pr.xs + ti * TX - H;
+    const int gyo = pr.ys + tj * TY - H;
+    const int W = pr.W, Hh = pr.H, P = pr.P;
+    const int gx = gxo + 2 * lane;
+    const bool colin = gx >= 0 && gx < W;
+    const T* kk = (const T*)pr.k;
+    const T* lbp = (const T*)pr.lb;
+    const bool has_lb = lbp != nullptr;
+    const T* srcp = (const T*)pr.x[a.pass & 1];
+    const int rxe = (gx > 0) + (gx < W - 1), rxo = (gx + 1 > 0) + (gx + 1 < W - 1);
+    const int base = (gxo + gyo) & 1;
+    const int cb = (base + wv) & 1;
+    const int rx0 = cb ? rxo : rxe, rx1 = cb ? rxe : rxo;
+    const T d = (T)pr.d, decay = (T)pr.decay, omega = (T)pr.omega, omo = (T)pr.omo;
+
+    T b0[NR], b1[NR], l0[NR], l1[NR], a0[NR], a1[NR], v0[NR], v1[NR];
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int gy = gyo + wv + NW * i;
+        T ce = nan_of<T>(), co = nan_of<T>(), le = T(0), lo = T(0);
+        v0[i] = T(0); v1[i] = T(0);
+        if (colin && gy >= 0 && gy < Hh) {
+            const size_t off = (size_t)gy * P + gx;
+            const T2 c = *(const T2*)(kk + off);
+            ce = c.x; co = c.y;
+            if (has_lb) { const T2 l = *(const T2*)(lbp + off); le = l.x; lo = l.y; }
+            const T2 v = *(const T2*)(srcp + off);
+            v0[i] = v.x; v1[i] = v.y;
+        }
+        b0[i] = cb ? co : ce; b1[i] = cb ? ce : co;
+        l0[i] = cb ? lo : le; l1[i] = cb ? le : lo;
+        const int ry = (gy > 0) + (gy < Hh - 1);
+        a0[i] = omega * d / ((T)(rx0 + ry) * d + decay);
+        a1[i] = omega * d / ((T)(rx1 + ry) * d + decay);
+    }
+
+    int par;
+    if (!sweep_prologue(a, pr, p, lt, &par)) return;
+    T* dst = (T*)pr.x[par ^ 1];
+
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int y = wv + NW * i;
+        cs[y][0][lane] = v0[i];
+        cs[y][1][lane] = v1[i];
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int h = 0; h < 2 * S; h++) {
+        const int colour = h & 1;
+        const int lo = h + 1, hiY = RY - 2 - h, hiX = RX - 2 - h;
+        const int q = colour ^ cb;
+        const int x = 2 * lane + q;
+        const bool xin = x >= lo && x <= hiX;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const int y = wv + NW * i;
+            if (y < lo || y > hiY) continue;
+            const T bq = colour ? b1[i] : b0[i];
+            if (!xin || bq != bq) continue;
+            const T before = cs[y][q][lane];
+            const T w = q ? cs[y][0][lane] : cs[y][1][lane - 1];
+            const T e = q ? cs[y][0][lane + 1] : cs[y][1][lane];
+            const T sum = (w + e) + (cs[y - 1][q][lane] + cs[y + 1][q][lane]);
+            T after = __builtin_fma(omo, before, __builtin_fma(colour ? a1[i] : a0[i], sum, bq));
+            if (has_lb) {
+                const T l = colour ? l1[i] : l0[i];
+                after = after > l ? after : l;
+            }
+            cs[y][q][lane] = after;
+        }
+        __syncthreads();
+    }
+
+    if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= pr.x1) {
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const int y = wv + NW * i;
+            if (y < H || y >= H + TY) continue;
+            const int gy = gyo + y;
+            if (gy > pr.y1) continue;
             T2 v;
             v.x = cs[y][0][lane];
             v.y = cs[y][1][lane];
             *(T2*)(dst + (size_t)gy * P + gx) = v;
         }
     }
-
-    if (a.check) block_partial(a.partials + (a.pass & 1) * a.part_stride, t, r2, 0.0);
-    if (PLAIN && a.crit == CRIT_MAXCHANGE && a.mode == MODE_SWEEP)
-        block_moved<NW, S>(a.ctl[p], a.pass, mv);
 }
 
 // ---------------------------------------------------------------------------
@@ -557,12 +641,26 @@ __global__ __launch_bounds__(256) void half3d(PassArgs a, int colour) {
                 if (et.gz > 0) sum += xs[i - plane];
                 if (et.gz + 1 < D) sum += xs[i + plane];
                 const int real = real_count(x, et.gy, et.gz, W, Hh, D);
-                const T k2q = general ? kk2[i] : T(0);
-                const T bal = Arith<T>::balanced(d, sum, kq, k2q, (T)pr.source, general, real,
-                                                 decay);
-                T after = Arith<T>::relax(omo, omega, before, bal);
-                if (PLAIN) after = after > T(0) ? after : T(0);
-                else if (has_lb) { const T l = lbp[i]; after = after > l ? after : l; }
+                T after;
+                if (PLAIN) {
+                    // reference order (FP64, no contraction)
+                    T num = d * sum;
+                    if (general) {
+                        num = num + (T)pr.source;
+                        num = num + kq;
+                        num = num - kk2[i];
+                    } else {
+                        num = num + kq;
+                    }
+                    const T bal = num / ((T)real * d + decay);
+                    after = omo * before + omega * bal;
+                    after = after > T(0) ? after : T(0);
+                } else {
+                    // pre-scaled linear smoother: kq = omega r / A
+                    const T alpha = omega * d / ((T)real * d + decay);
+                    after = __builtin_fma(omo, before, __builtin_fma(alpha, sum, kq));
+                    if (has_lb) { const T l = lbp[i]; after = after > l ? after : l; }
+                }
                 xs[i] = after;
                 if (PLAIN) mv[0] = change_bits((double)__builtin_fabs(after - before));
             }
@@ -575,8 +673,8 @@ __global__ __launch_bounds__(256) void half3d(PassArgs a, int colour) {
 // ring slot 0 of the partials. Per free cell: b = (source + production) -
 // uptake (or the folded k), r = (b + d*sum) - A*c, projected to the natural
 // residual; b_eff adds the held neighbours. With write32 it also writes the
-// FP32 defect right-hand side r, the correction's lower bound -c (rounded
-// toward zero) and zeroes e[0] for the FP32 inner solve.
+// FP32 smoother's right-hand side omega*r/A, the correction's lower bound -c
+// (rounded toward zero) and zeroes e[0] for the FP32 inner solve.
 __global__ __launch_bounds__(256) void resid64(PassArgs a, int write32) {
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int p = a.tile_prob[t];
@@ -627,7 +725,8 @@ __global__ __launch_bounds__(256) void resid64(PassArgs a, int write32) {
             r2 += rt * rt;
             b2 += be * be;
             if (write32) {
-                pr.r32[i] = (float)r;
+                // the FP32 smoother's pre-scaled right-hand side omega r / A
+                pr.r32[i] = (float)(pr.omega * r / A);
                 pr.e0[i] = 0.0f;
                 if (pr.lb32) pr.lb32[i] = __double2float_rz(-ci);
             }

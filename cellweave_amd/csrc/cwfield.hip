@@ -69,14 +69,13 @@ cw_status device_init() {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// Sweep-tile shapes (interior TX x TY of a 128 x RY region, halo 2s).
-struct TileShape { int tx, ty; };
-TileShape sweep_shape(bool fp64, int s) {
-    if (fp64) return s == 1 ? TileShape{124, 28} : TileShape{120, 24};
-    if (s == 1) return {124, 28};
-    if (s == 2) return {120, 24};
-    return {112, 48};
-}
+// Sweep-tile shapes: s sweeps per launch, regions 128 x RY cells, NW waves.
+// The interior is (128 - 4s) x (RY - 4s). Shapes 0 and 1 also exist for the
+// FP64 plain sweep.
+struct Shape { int s, ry, nw; };
+constexpr Shape kShapes[] = {{1, 32, 4}, {2, 32, 4}, {4, 64, 8}, {4, 64, 4}, {8, 128, 8}, {8, 128, 16}};
+constexpr int kNumShapes = 6;
+int default_shape(int s) { return s <= 1 ? 0 : s == 2 ? 1 : s <= 4 ? 2 : 4; }
 
 }  // namespace
 
@@ -213,8 +212,10 @@ struct Plan {
     hipStream_t stream = nullptr;
     int nprob = 0;
     int s_main = 1;
+    int shape = 0;       // kShapes index of the main sweep launch
     bool fp64 = true;
     bool plain = true;   // FP64: floored field (false: the unfloored adjoint)
+    bool general = false; // some problem keeps production and uptake apart
     bool three_d = false;
     std::vector<Prob> hp;
     std::vector<Ctl> hc;
@@ -260,8 +261,9 @@ cw_status plan_upload(Plan& pl) {
             } else if (pl.three_d) {
                 nx = ny = nz = 0;
             } else {
-                const TileShape ts = sweep_shape(pl.fp64, tl == TL_MAIN ? pl.s_main : 1);
-                nx = (sx + ts.tx - 1) / ts.tx; ny = (sy + ts.ty - 1) / ts.ty; nz = 1;
+                const Shape sh = kShapes[tl == TL_MAIN ? pl.shape : 0];
+                const int tx = 128 - 4 * sh.s, ty = sh.ry - 4 * sh.s;
+                nx = (sx + tx - 1) / tx; ny = (sy + ty - 1) / ty; nz = 1;
             }
             const int base = (int)pl.tp[tl].size(), count = nx * ny * nz;
             if (tl == TL_EL) {
@@ -315,20 +317,29 @@ PassArgs make_args(Plan& pl, int tl, int mode, int crit, double tol, int max_swe
 cw_status launch_sweep(Plan& pl, const PassArgs& a) {
     const int nt = (int)pl.tp[a.tl].size();
     if (nt == 0) return CW_OK;
-    const int s = a.s;
-    dim3 grid(nt);
-    if (pl.fp64) {
-        if (s == 1) {
-            if (pl.plain) hipLaunchKernelGGL((sweep2d<double, 1, 32, 4, true>), grid, dim3(256), 0, pl.stream, a);
-            else hipLaunchKernelGGL((sweep2d<double, 1, 32, 4, false>), grid, dim3(256), 0, pl.stream, a);
+    const int shape = a.tl == TL_MAIN ? pl.shape : 0;
+    const dim3 grid(nt), b256(256), b512(512), b1024(1024);
+    hipStream_t st = pl.stream;
+    if (pl.fp64 && pl.plain) {
+        if (pl.general) {
+            if (shape == 0) hipLaunchKernelGGL((sweep2d<1, 32, 4, true>), grid, b256, 0, st, a);
+            else hipLaunchKernelGGL((sweep2d<2, 32, 4, true>), grid, b256, 0, st, a);
         } else {
-            if (pl.plain) hipLaunchKernelGGL((sweep2d<double, 2, 32, 4, true>), grid, dim3(256), 0, pl.stream, a);
-            else hipLaunchKernelGGL((sweep2d<double, 2, 32, 4, false>), grid, dim3(256), 0, pl.stream, a);
+            if (shape == 0) hipLaunchKernelGGL((sweep2d<1, 32, 4, false>), grid, b256, 0, st, a);
+            else hipLaunchKernelGGL((sweep2d<2, 32, 4, false>), grid, b256, 0, st, a);
         }
+    } else if (pl.fp64) {
+        if (shape == 0) hipLaunchKernelGGL((smooth2d<double, 1, 32, 4>), grid, b256, 0, st, a);
+        else hipLaunchKernelGGL((smooth2d<double, 2, 32, 4>), grid, b256, 0, st, a);
     } else {
-        if (s == 1) hipLaunchKernelGGL((sweep2d<float, 1, 32, 4, false>), grid, dim3(256), 0, pl.stream, a);
-        else if (s == 2) hipLaunchKernelGGL((sweep2d<float, 2, 32, 4, false>), grid, dim3(256), 0, pl.stream, a);
-        else hipLaunchKernelGGL((sweep2d<float, 4, 64, 8, false>), grid, dim3(512), 0, pl.stream, a);
+        switch (shape) {
+        case 0: hipLaunchKernelGGL((smooth2d<float, 1, 32, 4>), grid, b256, 0, st, a); break;
+        case 1: hipLaunchKernelGGL((smooth2d<float, 2, 32, 4>), grid, b256, 0, st, a); break;
+        case 2: hipLaunchKernelGGL((smooth2d<float, 4, 64, 8>), grid, b512, 0, st, a); break;
+        case 3: hipLaunchKernelGGL((smooth2d<float, 4, 64, 4>), grid, b256, 0, st, a); break;
+        case 4: hipLaunchKernelGGL((smooth2d<float, 8, 128, 8>), grid, b512, 0, st, a); break;
+        default: hipLaunchKernelGGL((smooth2d<float, 8, 128, 16>), grid, b1024, 0, st, a); break;
+        }
     }
     CW_HIP(hipGetLastError());
     return CW_OK;
@@ -482,6 +493,7 @@ struct Opts {
     int check_every;
     double inner_tol;
     uint64_t max_outer;
+    int shape = -1;            // kShapes index forced for the smoother (-1: from s)
     long time_passes = 0;      // > 0: time this many sweep launches instead of solving
     double* time_ms = nullptr; // average duration of one sweep launch (HIP events)
 };
@@ -519,8 +531,13 @@ cw_status check_opts(const cw_relax_opts* o, Opts* out) {
     if (!(o->tolerance == o->tolerance)) return fail(CW_ERR_ARG, "tolerance is NaN");
     if (o->omega > 0.0 && !(o->omega < 2.0))
         return fail(CW_ERR_ARG, "omega %g outside (0, 2)", o->omega);
-    if (o->sweeps_per_pass < 0 || o->sweeps_per_pass > 4 || o->sweeps_per_pass == 3)
-        return fail(CW_ERR_ARG, "sweeps_per_pass must be 0, 1, 2 or 4");
+    if (o->sweeps_per_pass < 0 || o->sweeps_per_pass > 8 ||
+        (o->sweeps_per_pass & (o->sweeps_per_pass - 1)) != 0)
+        return fail(CW_ERR_ARG, "sweeps_per_pass must be 0, 1, 2, 4 or 8");
+    if (o->tile_shape >= kNumShapes) return fail(CW_ERR_ARG, "unknown tile shape %d", o->tile_shape);
+    if (o->tile_shape >= 0 && o->sweeps_per_pass && kShapes[o->tile_shape].s != o->sweeps_per_pass)
+        return fail(CW_ERR_ARG, "tile shape %d does not do %d sweeps per pass", o->tile_shape,
+                    o->sweeps_per_pass);
     out->omega = o->omega;
     out->crit = o->criterion == CW_CRIT_MAX_CHANGE ? CRIT_MAXCHANGE : CRIT_RESID;
     out->tol = o->tolerance;
@@ -530,6 +547,7 @@ cw_status check_opts(const cw_relax_opts* o, Opts* out) {
     out->check_every = o->check_every > 0 ? o->check_every : 0;
     out->inner_tol = o->inner_tolerance > 0.0 ? o->inner_tolerance : 1e-3;
     out->max_outer = o->max_outer > 0 ? o->max_outer : 1000;
+    out->shape = o->tile_shape;
     return CW_OK;
 }
 
@@ -553,6 +571,7 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     pl.fp64 = true;
     pl.three_d = three_d;
     pl.s_main = s;
+    pl.shape = default_shape(s);
     std::vector<int> idx;  // plan problem -> field
     std::vector<double> omegas(nf);
     for (uint32_t i = 0; i < nf; i++) {
@@ -581,6 +600,7 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
         pr.x[1] = three_d ? f->c[f->cur] : f->c[f->cur ^ 1];
         pr.k = f->k;
         pr.k2 = f->general ? f->k2 : nullptr;
+        if (f->general) pl.general = true;
         pr.c64 = f->c[f->cur];
         pl.hp.push_back(pr);
         Ctl c;
@@ -675,8 +695,9 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     for (uint32_t i = 0; i < nf; i++)
         if ((F[i]->D > 1) != three_d)
             return fail(CW_ERR_ARG, "a batch mixes planes and volumes");
-    int s = o.s ? o.s : 4;
-    if (three_d) s = 1;
+    int shape = o.shape >= 0 ? o.shape : default_shape(o.s ? o.s : 4);
+    if (three_d) shape = 0;
+    const int s = kShapes[shape].s;
     Plan outer, inner;
     outer.stream = inner.stream = stream;
     outer.fp64 = true;
@@ -684,6 +705,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     outer.three_d = inner.three_d = three_d;
     outer.s_main = 1;
     inner.s_main = s;
+    inner.shape = shape;
     std::vector<int> idx;
     std::vector<double> omegas(nf);
     int longest = 2;
@@ -859,6 +881,7 @@ void cw_relax_opts_default(cw_relax_opts* o) {
     o->check_every = 0;
     o->inner_tolerance = 0.0;
     o->max_outer = 0;
+    o->tile_shape = -1;
 }
 
 const char* cw_last_error(void) { return g_err.c_str(); }

@@ -55,7 +55,7 @@ def _dptr(a):
 
 def make_opts(omega=0.0, criterion=CRIT_MAX_CHANGE, tolerance=1e-8, max_sweeps=2_000_000,
               precision=PREC_FP64, sweeps_per_pass=0, check_every=0, inner_tolerance=0.0,
-              max_outer=0) -> RelaxOpts:
+              max_outer=0, tile_shape=-1) -> RelaxOpts:
     o = RelaxOpts()
     _abi.lib().cw_relax_opts_default(C.byref(o))
     o.omega = omega
@@ -67,6 +67,7 @@ def make_opts(omega=0.0, criterion=CRIT_MAX_CHANGE, tolerance=1e-8, max_sweeps=2
     o.check_every = check_every
     o.inner_tolerance = inner_tolerance
     o.max_outer = max_outer
+    o.tile_shape = tile_shape
     return o
 
 

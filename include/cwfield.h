@@ -67,6 +67,8 @@ typedef struct cw_relax_opts {
     int32_t check_every;      /* residual test every k passes; 0 = library choice */
     double inner_tolerance;   /* FP32_REFINE: inner reduction per outer step; 0 = default */
     uint64_t max_outer;       /* FP32_REFINE: outer iteration cap; 0 = default */
+    int32_t tile_shape;       /* FP32 smoother tile shape index (-1 = from
+                                 sweeps_per_pass); cw_autotune picks it */
 } cw_relax_opts;
 
 typedef struct cw_relax_stats {
