@@ -219,7 +219,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": f"sweep2d<float,{spp_used}> (FP32 smoother, {spp_used} sweeps per launch)",
+            "kernel": f"relax2d_reg<float,{spp_used},...> (FP32 smoother, {spp_used} sweeps per launch)",
             "alg_bytes_per_launch": bytes32, "launch_ms": ms32,
         },
         "cpu_baseline": cpu,

@@ -272,6 +272,18 @@ __device__ __forceinline__ int real_count(int gx, int gy, int gz, int W, int H, 
     return (gx > 0) + (gx < W - 1) + (gy > 0) + (gy < H - 1) + (gz > 0) + (gz < D - 1);
 }
 
+// Typed fused multiply-add (the untyped builtin would widen floats to double)
+// and the reciprocal used by the smoothers' coefficients.
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float max_t(float a, float b) { return __builtin_fmaxf(a, b); }
+__device__ __forceinline__ double max_t(double a, double b) { return __builtin_fmax(a, b); }
+template <typename T> __device__ __forceinline__ T neg_inf();
+template <> __device__ __forceinline__ float neg_inf<float>() { return -__builtin_huge_valf(); }
+template <> __device__ __forceinline__ double neg_inf<double>() { return -__builtin_huge_val(); }
+__device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
+
 template <typename T> __device__ __forceinline__ T nan_of();
 template <> __device__ __forceinline__ double nan_of<double>() {
     return __longlong_as_double(0x7ff8000000000000ll);
@@ -522,8 +534,8 @@ __global__ __launch_bounds__(NW * 64) void smooth2d(PassArgs a) {
         b0[i] = cb ? co : ce; b1[i] = cb ? ce : co;
         l0[i] = cb ? lo : le; l1[i] = cb ? le : lo;
         const int ry = (gy > 0) + (gy < Hh - 1);
-        a0[i] = omega * d / ((T)(rx0 + ry) * d + decay);
-        a1[i] = omega * d / ((T)(rx1 + ry) * d + decay);
+        a0[i] = omega * d * rcp_t((T)(rx0 + ry) * d + decay);
+        a1[i] = omega * d * rcp_t((T)(rx1 + ry) * d + decay);
     }
 
     int par;
@@ -555,7 +567,7 @@ __global__ __launch_bounds__(NW * 64) void smooth2d(PassArgs a) {
             const T w = q ? cs[y][0][lane] : cs[y][1][lane - 1];
             const T e = q ? cs[y][0][lane + 1] : cs[y][1][lane];
             const T sum = (w + e) + (cs[y - 1][q][lane] + cs[y + 1][q][lane]);
-            T after = __builtin_fma(omo, before, __builtin_fma(colour ? a1[i] : a0[i], sum, bq));
+            T after = fma_t(omo, before, fma_t(colour ? a1[i] : a0[i], sum, bq));
             if (has_lb) {
                 const T l = colour ? l1[i] : l0[i];
                 after = after > l ? after : l;
@@ -577,6 +589,452 @@ __global__ __launch_bounds__(NW * 64) void smooth2d(PassArgs a) {
             v.y = cs[y][1][lane];
             *(T2*)(dst + (size_t)gy * P + gx) = v;
         }
+    }
+}
+
+// Neighbour values across lanes without LDS: DPP wave shifts (GFX9 family).
+// from_left gives lane l the value of lane l-1, from_right that of lane l+1;
+// the wave's end lanes get 0 (their cells sit on the region's edge and are
+// never updated).
+__device__ __forceinline__ float from_left(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double from_left(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), 0x138, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double from_right(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), 0x130, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// Register-blocked linear smoother, the same update as smooth2d. Wave w owns
+// the contiguous rows [w*NR, (w+1)*NR) of a 128 x (NW*NR) region and lane l
+// the columns (2l, 2l+1) of each: the lane's 2*NR cells live in registers for
+// the whole launch. Vertical neighbours are the lane's own registers, the
+// left or right neighbour across lanes comes by a DPP wave shift, and only a
+// wave's first and last rows go through LDS, to the waves above and below
+// (double-buffered by half-sweep parity: one barrier per half sweep).
+template <typename T, int S, int NW, int NR>
+__global__ __launch_bounds__(NW * 64) void smooth2d_reg(PassArgs a) {
+    constexpr int H = 2 * S;
+    constexpr int RX = 128;
+    constexpr int RY = NW * NR;
+    constexpr int TX = RX - 2 * H;
+    constexpr int TY = RY - 2 * H;
+    static_assert(TY > 0 && NR >= 2, "region shape");
+    using T2 = typename V2<T>::type;
+    __shared__ T edge[2][NW][2][2][64];  // [slot][wave][first/last row][column parity][lane]
+
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int p = a.tile_prob[t];
+    const Prob& pr = a.probs[p];
+    const int tl = a.tl;
+    const int lt = t - pr.sw_base[tl];
+    const int ti = lt % pr.ntx[tl], tj = lt / pr.ntx[tl];
+    const int gxo = pr.xs + ti * TX - H;
+    const int gyo = pr.ys + tj * TY - H;
+    const int W = pr.W, Hh = pr.H, P = pr.P;
+    const int gx = gxo + 2 * lane;
+    const bool colin = gx >= 0 && gx < W;
+    const T* kk = (const T*)pr.k;
+    const T* lbp = (const T*)pr.lb;
+    const bool has_lb = lbp != nullptr;
+    const T* srcp = (const T*)pr.x[a.pass & 1];
+    const T d = (T)pr.d, decay = (T)pr.decay, omega = (T)pr.omega, omo = (T)pr.omo;
+    const int y0 = wv * NR;  // first region row of this wave
+
+    // alpha = omega d / (real d + decay) by the real neighbour count in x of
+    // each column (rx, per lane) and in y of the row (ry = 0, 1, 2, uniform).
+    const int rxe = (gx > 0) + (gx < W - 1), rxo = (gx + 1 > 0) + (gx + 1 < W - 1);
+    const T od = omega * d;
+    const T ae0 = od * rcp_t((T)rxe * d + decay), ae1 = od * rcp_t((T)(rxe + 1) * d + decay),
+            ae2 = od * rcp_t((T)(rxe + 2) * d + decay);
+    const T ao0 = od * rcp_t((T)rxo * d + decay), ao1 = od * rcp_t((T)(rxo + 1) * d + decay),
+            ao2 = od * rcp_t((T)(rxo + 2) * d + decay);
+
+    T ve[NR], vo[NR], be[NR], bo[NR], le[NR], lo[NR];
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int gy = gyo + y0 + i;
+        ve[i] = T(0); vo[i] = T(0); be[i] = nan_of<T>(); bo[i] = nan_of<T>();
+        le[i] = T(0); lo[i] = T(0);
+        if (colin && gy >= 0 && gy < Hh) {
+            const size_t off = (size_t)gy * P + gx;
+            const T2 c = *(const T2*)(kk + off);
+            be[i] = c.x; bo[i] = c.y;
+            if (has_lb) { const T2 l = *(const T2*)(lbp + off); le[i] = l.x; lo[i] = l.y; }
+            const T2 v = *(const T2*)(srcp + off);
+            ve[i] = v.x; vo[i] = v.y;
+        }
+    }
+
+    int par;
+    if (!sweep_prologue(a, pr, p, lt, &par)) return;
+    T* dst = (T*)pr.x[par ^ 1];
+
+    // publish the edge rows for half sweep 0
+    edge[0][wv][0][0][lane] = ve[0];
+    edge[0][wv][0][1][lane] = vo[0];
+    edge[0][wv][1][0][lane] = ve[NR - 1];
+    edge[0][wv][1][1][lane] = vo[NR - 1];
+    __syncthreads();
+
+    const int base = (gxo + gyo + y0) & 1;  // colour of column 2l on the wave's first row
+#pragma unroll
+    for (int h = 0; h < 2 * S; h++) {
+        const int slot = h & 1;
+        const int colour = h & 1;
+        const int lo_ = h + 1, hiY = RY - 2 - h, hiX = RX - 2 - h;
+        // q0: column parity updated on the wave's first row; row i updates q0 ^ (i & 1)
+        const int q0 = (colour + base) & 1;
+        const bool ein = 2 * lane >= lo_ && 2 * lane <= hiX;
+        const bool oin = 2 * lane + 1 >= lo_ && 2 * lane + 1 <= hiX;
+        // halo rows above and below the wave's block (other colour, current values)
+        T above = T(0), below = T(0);
+        {
+            const int qa = q0;                 // column updated on row 0
+            const int qb = q0 ^ ((NR - 1) & 1); // column updated on row NR-1
+            if (wv > 0) above = edge[slot][wv - 1][1][qa][lane];
+            if (wv < NW - 1) below = edge[slot][wv + 1][0][qb][lane];
+        }
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const int r = y0 + i;
+            if (r < lo_ || r > hiY) continue;       // uniform per wave
+            const int gy = gyo + r;
+            const int ry = (gy > 0) + (gy < Hh - 1);
+            const int q = q0 ^ (i & 1);             // uniform per wave
+            // neighbours of column q on row i
+            const T up = i > 0 ? (q ? vo[i - 1] : ve[i - 1]) : above;
+            const T dn = i < NR - 1 ? (q ? vo[i + 1] : ve[i + 1]) : below;
+            if (q == 0) {
+                const T left = from_left(vo[i]);
+                const T sum = (left + vo[i]) + (up + dn);
+                const T al = ry == 2 ? ae2 : (ry == 1 ? ae1 : ae0);
+                T after = fma_t(omo, ve[i], fma_t(al, sum, be[i]));
+                if (has_lb) after = after > le[i] ? after : le[i];
+                const bool upd = ein && be[i] == be[i];
+                ve[i] = upd ? after : ve[i];
+            } else {
+                const T right = from_right(ve[i]);
+                const T sum = (ve[i] + right) + (up + dn);
+                const T al = ry == 2 ? ao2 : (ry == 1 ? ao1 : ao0);
+                T after = fma_t(omo, vo[i], fma_t(al, sum, bo[i]));
+                if (has_lb) after = after > lo[i] ? after : lo[i];
+                const bool upd = oin && bo[i] == bo[i];
+                vo[i] = upd ? after : vo[i];
+            }
+        }
+        if (h + 1 < 2 * S) {
+            const int ns = slot ^ 1;
+            edge[ns][wv][0][0][lane] = ve[0];
+            edge[ns][wv][0][1][lane] = vo[0];
+            edge[ns][wv][1][0][lane] = ve[NR - 1];
+            edge[ns][wv][1][1][lane] = vo[NR - 1];
+            __syncthreads();
+        }
+    }
+
+    if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= pr.x1) {
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const int r = y0 + i;
+            if (r < H || r >= H + TY) continue;
+            const int gy = gyo + r;
+            if (gy > pr.y1) continue;
+            T2 v;
+            v.x = ve[i];
+            v.y = vo[i];
+            *(T2*)(dst + (size_t)gy * P + gx) = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Register-blocked red-black sweep (2D), for both forms:
+//   PLAIN: the reference's FP64 update (pde.rs:353-398) bit for bit, with the
+//          floor, the per-sweep largest change and the optional residual of
+//          the input; GEN keeps production and uptake apart.
+//   else:  the pre-scaled linear smoother e <- max(lb, (1-w) e + a sum + b).
+// Wave w owns the contiguous rows [w*NR, (w+1)*NR) of a 128 x (NW*NR)
+// region; lane l owns columns (2l, 2l+1) of each row. The lane keeps its
+// cells in registers by colour: st[c][i] is the colour-c cell of row i. For a
+// colour-c cell, the vertical neighbours are st[1-c][i-1] and st[1-c][i+1]
+// (same column, other colour), one horizontal neighbour is st[1-c][i] (the
+// lane's other cell) and the other is the neighbouring lane's st[1-c][i],
+// fetched by a DPP wave shift. The x parity of colour c on row i is
+// P0 ^ (i & 1) with P0 wave-uniform: one branch per half sweep picks an
+// instantiation in which every row's parity is a compile-time constant. Only
+// a wave's first and last rows go through LDS (double-buffered).
+// ---------------------------------------------------------------------------
+template <typename T, int NR>
+struct RegCells {
+    T st[2][NR];   // state by colour
+    T kc[2][NR];   // PLAIN: coefficient (NaN = held or outside); smoother: beta
+    T k2[2][NR];   // PLAIN+GEN: uptake; smoother: lower bound
+    T al[2][NR];   // smoother: alpha = omega d / A (0 on held or outside cells)
+    T ga[2][NR];   // smoother: gamma = -omega (0 on held or outside cells)
+};
+
+template <typename T, bool PLAIN, bool GEN, int NR, int C, int P0, int S>
+__device__ __forceinline__ void reg_half(RegCells<T, NR>& R, T above, T below, int r0, int lo,
+                                         int hiY, int hiX, int lane, int gyo, int Hh,
+                                         const int (&rx)[2], T d, T decay, T omega, T omo,
+                                         T source, const T (&al)[2][3], bool has_lb, int H,
+                                         int TX, int TY, unsigned long long* mv) {
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int q = P0 ^ (i & 1);  // compile-time after unrolling
+        const int r = r0 + i;
+        if (PLAIN && (r < lo || r > hiY)) continue;
+        const int x = 2 * lane + q;
+        const T o = R.st[C ^ 1][i];
+        const T across = q ? from_right(o) : from_left(o);
+        const T up = i > 0 ? R.st[C ^ 1][i - 1] : above;
+        const T dn = i < NR - 1 ? R.st[C ^ 1][i + 1] : below;
+        const T before = R.st[C][i];
+        T after;
+        if (PLAIN) {
+            const T kq = R.kc[C][i];
+            const bool upd = x >= lo && x <= hiX && kq == kq;
+            const int gy = gyo + r;
+            const int ry = (gy > 0) + (gy < Hh - 1);
+            // x-1, x+1, y-1, y+1, in the reference's order
+            T sum = T(0);
+            if (q == 0) { sum += across; sum += o; } else { sum += o; sum += across; }
+            sum += up;
+            sum += dn;
+            T num = d * sum;
+            if (GEN) {
+                num = num + source;
+                num = num + kq;
+                num = num - R.k2[C][i];
+            } else {
+                num = num + kq;
+            }
+            const T bal = num / ((T)(rx[q] + ry) * d + decay);
+            after = omo * before + omega * bal;
+            after = after > T(0) ? after : T(0);
+            if (upd && r >= H && r < H + TY && x >= H && x < H + TX && S > 0) {
+                const unsigned long long m = change_bits((double)__builtin_fabs(after - before));
+                *mv = m > *mv ? m : *mv;
+            }
+            R.st[C][i] = upd ? after : before;
+        } else {
+            // before + alpha*sum + (beta - omega*before): a held or outside
+            // cell has alpha = beta = gamma = 0 and keeps its value exactly.
+            // Cells of the stale halo are updated too: they are invalid at
+            // this depth whatever they hold, and never reach a valid cell.
+            const T sum = (across + o) + (up + dn);
+            after = before + fma_t(R.al[C][i], sum, fma_t(R.ga[C][i], before, R.kc[C][i]));
+            R.st[C][i] = max_t(after, R.k2[C][i]);  // lower bound (-inf without one)
+        }
+    }
+}
+
+template <typename T, int S, int NW, int NR, bool PLAIN, bool GEN>
+__global__ __launch_bounds__(NW * 64) void relax2d_reg(PassArgs a) {
+    constexpr int H = 2 * S;
+    constexpr int RX = 128;
+    constexpr int RY = NW * NR;
+    constexpr int TX = RX - 2 * H;
+    constexpr int TY = RY - 2 * H;
+    static_assert(TY > 0 && NR >= 2 && S <= kMaxS * 2, "region shape");
+    static_assert(!PLAIN || S <= kMaxS, "change slots");
+    using T2 = typename V2<T>::type;
+    __shared__ T edge[2][NW][2][2][64];  // [slot][wave][first/last row][colour][lane]
+
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int p = a.tile_prob[t];
+    const Prob& pr = a.probs[p];
+    const int tl = a.tl;
+    const int lt = t - pr.sw_base[tl];
+    const int ti = lt % pr.ntx[tl], tj = lt / pr.ntx[tl];
+    const int gxo = pr.xs + ti * TX - H;
+    const int gyo = pr.ys + tj * TY - H;
+    const int W = pr.W, Hh = pr.H, P = pr.P;
+    const int gx = gxo + 2 * lane;
+    const bool colin = gx >= 0 && gx < W;
+    const T* kk = (const T*)pr.k;
+    const T* k2p = (const T*)(PLAIN ? pr.k2 : pr.lb);
+    const bool has_k2 = PLAIN ? GEN : (k2p != nullptr);
+    const bool replay = a.mode == MODE_REPLAY;
+    const T d = (T)pr.d, decay = (T)pr.decay, omega = (T)pr.omega, omo = (T)pr.omo;
+    const T source = (T)pr.source;
+    const int r0 = wv * NR;
+    const int base = (gxo + gyo + r0) & 1;  // colour of column 2l on the wave's first row
+    const int rx[2] = {(gx > 0) + (gx < W - 1), (gx + 1 > 0) + (gx + 1 < W - 1)};
+    T al[2][3];
+    if (!PLAIN) {
+        const T od = omega * d;
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int ry = 0; ry < 3; ry++) al[q][ry] = od * rcp_t((T)(rx[q] + ry) * d + decay);
+    }
+
+    RegCells<T, NR> R;
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int gy = gyo + r0 + i;
+        T ve = T(0), vo = T(0), ce = nan_of<T>(), co = nan_of<T>(), ue = T(0), uo = T(0);
+        if (colin && gy >= 0 && gy < Hh) {
+            const size_t off = (size_t)gy * P + gx;
+            const T2 c = *(const T2*)(kk + off);
+            ce = c.x; co = c.y;
+            if (has_k2) { const T2 u = *(const T2*)(k2p + off); ue = u.x; uo = u.y; }
+            if (!replay) {
+                const T2 v = *(const T2*)((const T*)pr.x[a.pass & 1] + off);
+                ve = v.x; vo = v.y;
+            }
+        }
+        // colour of the even column on row i is base ^ (i & 1)
+        const bool sw = (base ^ (i & 1)) != 0;
+        R.st[0][i] = sw ? vo : ve; R.st[1][i] = sw ? ve : vo;
+        R.kc[0][i] = sw ? co : ce; R.kc[1][i] = sw ? ce : co;
+        R.k2[0][i] = sw ? uo : ue; R.k2[1][i] = sw ? ue : uo;
+        if (!PLAIN) {
+            const int ry = (gy > 0) + (gy < Hh - 1);
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int q = (c ^ base ^ (i & 1)) & 1;  // x parity of colour c on row i
+                const T b = R.kc[c][i];
+                const bool fr = b == b;
+                R.al[c][i] = fr ? al[q][ry] : T(0);
+                R.ga[c][i] = fr ? -omega : T(0);
+                R.kc[c][i] = fr ? b : T(0);
+                if (!has_k2) R.k2[c][i] = neg_inf<T>();
+            }
+        }
+    }
+
+    int par;
+    if (!sweep_prologue(a, pr, p, lt, &par)) return;
+    const T* src = (const T*)pr.x[par];
+    T* dst = (T*)pr.x[par ^ 1];
+    if (replay) {
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const int gy = gyo + r0 + i;
+            if (colin && gy >= 0 && gy < Hh) {
+                const T2 v = *(const T2*)(src + (size_t)gy * P + gx);
+                const bool sw = (base ^ (i & 1)) != 0;
+                R.st[0][i] = sw ? v.y : v.x; R.st[1][i] = sw ? v.x : v.y;
+            }
+        }
+    }
+
+    // Residual of the input on the interior (PLAIN, relative-residual rule).
+    double r2 = 0.0;
+    if (PLAIN && a.check) {
+        edge[0][wv][0][0][lane] = R.st[0][0];
+        edge[0][wv][0][1][lane] = R.st[1][0];
+        edge[0][wv][1][0][lane] = R.st[0][NR - 1];
+        edge[0][wv][1][1][lane] = R.st[1][NR - 1];
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const T above = wv > 0 ? edge[0][wv - 1][1][c ^ 1][lane] : T(0);
+            const T below = wv < NW - 1 ? edge[0][wv + 1][0][c ^ 1][lane] : T(0);
+#pragma unroll
+            for (int i = 0; i < NR; i++) {
+                const int r = r0 + i;
+                if (r < H || r >= H + TY) continue;
+                const int q = (c ^ base ^ (i & 1)) & 1;
+                const int x = 2 * lane + q;
+                const T kq = R.kc[c][i];
+                const T o = R.st[c ^ 1][i];
+                const T fl = from_left(o), fr = from_right(o);
+                if (x < H || x >= H + TX || kq != kq) continue;
+                const T across = q ? fr : fl;
+                const T up = i > 0 ? R.st[c ^ 1][i - 1] : above;
+                const T dn = i < NR - 1 ? R.st[c ^ 1][i + 1] : below;
+                T sum = T(0);
+                if (q == 0) { sum += across; sum += o; } else { sum += o; sum += across; }
+                sum += up;
+                sum += dn;
+                const int gy = gyo + r;
+                const int ry = (gy > 0) + (gy < Hh - 1);
+                const double A = (double)(rx[q] + ry) * (double)d + (double)decay;
+                const double b = GEN ? (((double)source + (double)kq) - (double)R.k2[c][i])
+                                     : (double)kq;
+                const double cc = (double)R.st[c][i];
+                double rr = (b + (double)d * (double)sum) - A * cc;
+                if (A > 0.0 && !(cc + rr / A > 0.0)) rr = -A * cc;
+                r2 += rr * rr;
+            }
+        }
+        __syncthreads();
+    }
+
+    edge[0][wv][0][0][lane] = R.st[0][0];
+    edge[0][wv][0][1][lane] = R.st[1][0];
+    edge[0][wv][1][0][lane] = R.st[0][NR - 1];
+    edge[0][wv][1][1][lane] = R.st[1][NR - 1];
+    __syncthreads();
+
+    unsigned long long mv[S > kMaxS ? 1 : S];
+#pragma unroll
+    for (int j = 0; j < (S > kMaxS ? 1 : S); j++) mv[j] = 0ull;
+    const bool has_lb = !PLAIN && has_k2;
+#pragma unroll
+    for (int h = 0; h < 2 * S; h++) {
+        const int slot = h & 1;
+        const int lo = h + 1, hiY = RY - 2 - h, hiX = RX - 2 - h;
+        // rows above and below hold the other colour at the same column
+        const int oc = (h & 1) ^ 1;
+        const T above = wv > 0 ? edge[slot][wv - 1][1][oc][lane] : T(0);
+        const T below = wv < NW - 1 ? edge[slot][wv + 1][0][oc][lane] : T(0);
+        unsigned long long* m = &mv[PLAIN ? (h >> 1) % (S > kMaxS ? 1 : S) : 0];
+        const int p0 = ((h & 1) ^ base) & 1;  // x parity of this colour on row 0
+        if (h & 1) {
+            if (p0) reg_half<T, PLAIN, GEN, NR, 1, 1, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+            else reg_half<T, PLAIN, GEN, NR, 1, 0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+        } else {
+            if (p0) reg_half<T, PLAIN, GEN, NR, 0, 1, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+            else reg_half<T, PLAIN, GEN, NR, 0, 0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+        }
+        if (h + 1 < 2 * S) {
+            const int ns = slot ^ 1;
+            edge[ns][wv][0][0][lane] = R.st[0][0];
+            edge[ns][wv][0][1][lane] = R.st[1][0];
+            edge[ns][wv][1][0][lane] = R.st[0][NR - 1];
+            edge[ns][wv][1][1][lane] = R.st[1][NR - 1];
+            __syncthreads();
+        }
+    }
+
+    if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= pr.x1) {
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const int r = r0 + i;
+            if (r < H || r >= H + TY) continue;
+            const int gy = gyo + r;
+            if (gy > pr.y1) continue;
+            const bool sw = (base ^ (i & 1)) != 0;
+            T2 v;
+            v.x = sw ? R.st[1][i] : R.st[0][i];
+            v.y = sw ? R.st[0][i] : R.st[1][i];
+            *(T2*)(dst + (size_t)gy * P + gx) = v;
+        }
+    }
+
+    if (PLAIN) {
+        if (a.check) block_partial(a.partials + (a.pass & 1) * a.part_stride, t, r2, 0.0);
+        if (a.crit == CRIT_MAXCHANGE && a.mode == MODE_SWEEP)
+            block_moved<NW, (S > kMaxS ? 1 : S)>(a.ctl[p], a.pass, mv);
     }
 }
 
@@ -657,8 +1115,8 @@ __global__ __launch_bounds__(256) void half3d(PassArgs a, int colour) {
                     after = after > T(0) ? after : T(0);
                 } else {
                     // pre-scaled linear smoother: kq = omega r / A
-                    const T alpha = omega * d / ((T)real * d + decay);
-                    after = __builtin_fma(omo, before, __builtin_fma(alpha, sum, kq));
+                    const T alpha = omega * d * rcp_t((T)real * d + decay);
+                    after = fma_t(omo, before, fma_t(alpha, sum, kq));
                     if (has_lb) { const T l = lbp[i]; after = after > l ? after : l; }
                 }
                 xs[i] = after;

@@ -69,13 +69,23 @@ cw_status device_init() {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// Sweep-tile shapes: s sweeps per launch, regions 128 x RY cells, NW waves.
-// The interior is (128 - 4s) x (RY - 4s). Shapes 0 and 1 also exist for the
-// FP64 plain sweep.
-struct Shape { int s, ry, nw; };
-constexpr Shape kShapes[] = {{1, 32, 4}, {2, 32, 4}, {4, 64, 8}, {4, 64, 4}, {8, 128, 8}, {8, 128, 16}};
-constexpr int kNumShapes = 6;
-int default_shape(int s) { return s <= 1 ? 0 : s == 2 ? 1 : s <= 4 ? 2 : 4; }
+// Sweep-tile shapes: s sweeps per launch over regions 128 x RY cells with NW
+// waves; the interior is (128 - 4s) x (RY - 4s). kind 0: LDS tile kernel;
+// 1: register-blocked by column parity; 2: register-blocked by colour
+// (relax2d_reg, NR = RY / NW rows per wave). Index 0 of each table is the
+// one-sweep shape used for remainders and exact-stop replays.
+struct Shape { int s, ry, nw, kind; };
+constexpr Shape kPlainShapes[] = {{1, 32, 4, 0}, {2, 32, 4, 0},  {1, 32, 4, 2}, {2, 32, 4, 2},
+                                  {2, 64, 8, 2}, {2, 64, 16, 2}, {1, 32, 8, 2}};
+constexpr Shape kSmoothShapes[] = {{1, 32, 4, 0},   {2, 32, 4, 0},   {4, 64, 8, 0},   {4, 64, 4, 0},
+                                   {8, 128, 8, 0},  {8, 128, 16, 0}, {2, 32, 4, 1},   {4, 64, 16, 1},
+                                   {4, 64, 8, 1},   {8, 128, 16, 1}, {2, 64, 8, 1},   {4, 128, 16, 1},
+                                   {2, 32, 4, 2},   {4, 64, 8, 2},   {4, 64, 16, 2},  {8, 128, 16, 2},
+                                   {2, 64, 8, 2},   {4, 128, 16, 2}};
+constexpr int kNumPlainShapes = (int)(sizeof(kPlainShapes) / sizeof(kPlainShapes[0]));
+constexpr int kNumSmoothShapes = (int)(sizeof(kSmoothShapes) / sizeof(kSmoothShapes[0]));
+int default_plain_shape(int s) { return s <= 1 ? 2 : 3; }
+int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 : 15; }
 
 }  // namespace
 
@@ -261,7 +271,8 @@ cw_status plan_upload(Plan& pl) {
             } else if (pl.three_d) {
                 nx = ny = nz = 0;
             } else {
-                const Shape sh = kShapes[tl == TL_MAIN ? pl.shape : 0];
+                const int si = tl == TL_MAIN ? pl.shape : 0;
+                const Shape sh = (pl.fp64 && pl.plain) ? kPlainShapes[si] : kSmoothShapes[si];
                 const int tx = 128 - 4 * sh.s, ty = sh.ry - 4 * sh.s;
                 nx = (sx + tx - 1) / tx; ny = (sy + ty - 1) / ty; nz = 1;
             }
@@ -320,27 +331,58 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a) {
     const int shape = a.tl == TL_MAIN ? pl.shape : 0;
     const dim3 grid(nt), b256(256), b512(512), b1024(1024);
     hipStream_t st = pl.stream;
+#define CW_L(kern, blk) hipLaunchKernelGGL(kern, grid, blk, 0, st, a)
     if (pl.fp64 && pl.plain) {
         if (pl.general) {
-            if (shape == 0) hipLaunchKernelGGL((sweep2d<1, 32, 4, true>), grid, b256, 0, st, a);
-            else hipLaunchKernelGGL((sweep2d<2, 32, 4, true>), grid, b256, 0, st, a);
+            switch (shape) {
+            case 0: CW_L((sweep2d<1, 32, 4, true>), b256); break;
+            case 1: CW_L((sweep2d<2, 32, 4, true>), b256); break;
+            case 2: CW_L((relax2d_reg<double, 1, 4, 8, true, true>), b256); break;
+            case 3: CW_L((relax2d_reg<double, 2, 4, 8, true, true>), b256); break;
+            case 4: CW_L((relax2d_reg<double, 2, 8, 8, true, true>), b512); break;
+            case 5: CW_L((relax2d_reg<double, 2, 16, 4, true, true>), b1024); break;
+            default: CW_L((relax2d_reg<double, 1, 8, 4, true, true>), b512); break;
+            }
         } else {
-            if (shape == 0) hipLaunchKernelGGL((sweep2d<1, 32, 4, false>), grid, b256, 0, st, a);
-            else hipLaunchKernelGGL((sweep2d<2, 32, 4, false>), grid, b256, 0, st, a);
+            switch (shape) {
+            case 0: CW_L((sweep2d<1, 32, 4, false>), b256); break;
+            case 1: CW_L((sweep2d<2, 32, 4, false>), b256); break;
+            case 2: CW_L((relax2d_reg<double, 1, 4, 8, true, false>), b256); break;
+            case 3: CW_L((relax2d_reg<double, 2, 4, 8, true, false>), b256); break;
+            case 4: CW_L((relax2d_reg<double, 2, 8, 8, true, false>), b512); break;
+            case 5: CW_L((relax2d_reg<double, 2, 16, 4, true, false>), b1024); break;
+            default: CW_L((relax2d_reg<double, 1, 8, 4, true, false>), b512); break;
+            }
         }
     } else if (pl.fp64) {
-        if (shape == 0) hipLaunchKernelGGL((smooth2d<double, 1, 32, 4>), grid, b256, 0, st, a);
-        else hipLaunchKernelGGL((smooth2d<double, 2, 32, 4>), grid, b256, 0, st, a);
+        switch (shape) {
+        case 0: CW_L((smooth2d<double, 1, 32, 4>), b256); break;
+        case 1: CW_L((smooth2d<double, 2, 32, 4>), b256); break;
+        default: CW_L((relax2d_reg<double, 2, 4, 8, false, false>), b256); break;
+        }
     } else {
         switch (shape) {
-        case 0: hipLaunchKernelGGL((smooth2d<float, 1, 32, 4>), grid, b256, 0, st, a); break;
-        case 1: hipLaunchKernelGGL((smooth2d<float, 2, 32, 4>), grid, b256, 0, st, a); break;
-        case 2: hipLaunchKernelGGL((smooth2d<float, 4, 64, 8>), grid, b512, 0, st, a); break;
-        case 3: hipLaunchKernelGGL((smooth2d<float, 4, 64, 4>), grid, b256, 0, st, a); break;
-        case 4: hipLaunchKernelGGL((smooth2d<float, 8, 128, 8>), grid, b512, 0, st, a); break;
-        default: hipLaunchKernelGGL((smooth2d<float, 8, 128, 16>), grid, b1024, 0, st, a); break;
+        case 0: CW_L((smooth2d<float, 1, 32, 4>), b256); break;
+        case 1: CW_L((smooth2d<float, 2, 32, 4>), b256); break;
+        case 2: CW_L((smooth2d<float, 4, 64, 8>), b512); break;
+        case 3: CW_L((smooth2d<float, 4, 64, 4>), b256); break;
+        case 4: CW_L((smooth2d<float, 8, 128, 8>), b512); break;
+        case 5: CW_L((smooth2d<float, 8, 128, 16>), b1024); break;
+        case 6: CW_L((smooth2d_reg<float, 2, 4, 8>), b256); break;
+        case 7: CW_L((smooth2d_reg<float, 4, 16, 4>), b1024); break;
+        case 8: CW_L((smooth2d_reg<float, 4, 8, 8>), b512); break;
+        case 9: CW_L((smooth2d_reg<float, 8, 16, 8>), b1024); break;
+        case 10: CW_L((smooth2d_reg<float, 2, 8, 8>), b512); break;
+        case 11: CW_L((smooth2d_reg<float, 4, 16, 8>), b1024); break;
+        case 12: CW_L((relax2d_reg<float, 2, 4, 8, false, false>), b256); break;
+        case 13: CW_L((relax2d_reg<float, 4, 8, 8, false, false>), b512); break;
+        case 14: CW_L((relax2d_reg<float, 4, 16, 4, false, false>), b1024); break;
+        case 15: CW_L((relax2d_reg<float, 8, 16, 8, false, false>), b1024); break;
+        case 16: CW_L((relax2d_reg<float, 2, 8, 8, false, false>), b512); break;
+        default: CW_L((relax2d_reg<float, 4, 16, 8, false, false>), b1024); break;
         }
     }
+#undef CW_L
     CW_HIP(hipGetLastError());
     return CW_OK;
 }
@@ -534,10 +576,18 @@ cw_status check_opts(const cw_relax_opts* o, Opts* out) {
     if (o->sweeps_per_pass < 0 || o->sweeps_per_pass > 8 ||
         (o->sweeps_per_pass & (o->sweeps_per_pass - 1)) != 0)
         return fail(CW_ERR_ARG, "sweeps_per_pass must be 0, 1, 2, 4 or 8");
-    if (o->tile_shape >= kNumShapes) return fail(CW_ERR_ARG, "unknown tile shape %d", o->tile_shape);
-    if (o->tile_shape >= 0 && o->sweeps_per_pass && kShapes[o->tile_shape].s != o->sweeps_per_pass)
-        return fail(CW_ERR_ARG, "tile shape %d does not do %d sweeps per pass", o->tile_shape,
-                    o->sweeps_per_pass);
+    {
+        const bool plain = o->precision == CW_PREC_FP64;
+        const int nshape = plain ? kNumPlainShapes : kNumSmoothShapes;
+        if (o->tile_shape >= nshape)
+            return fail(CW_ERR_ARG, "unknown tile shape %d", o->tile_shape);
+        const int ts = o->tile_shape < 0 ? 0
+                       : plain ? kPlainShapes[o->tile_shape].s
+                               : kSmoothShapes[o->tile_shape].s;
+        if (o->tile_shape >= 0 && o->sweeps_per_pass && ts != o->sweeps_per_pass)
+            return fail(CW_ERR_ARG, "tile shape %d does not do %d sweeps per pass", o->tile_shape,
+                        o->sweeps_per_pass);
+    }
     out->omega = o->omega;
     out->crit = o->criterion == CW_CRIT_MAX_CHANGE ? CRIT_MAXCHANGE : CRIT_RESID;
     out->tol = o->tolerance;
@@ -571,7 +621,10 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     pl.fp64 = true;
     pl.three_d = three_d;
     pl.s_main = s;
-    pl.shape = default_shape(s);
+    pl.shape = o.shape >= 0 ? o.shape : default_plain_shape(s);
+    if (pl.shape >= kNumPlainShapes) return fail(CW_ERR_ARG, "unknown tile shape %d", pl.shape);
+    pl.s_main = kPlainShapes[pl.shape].s;
+    s = pl.s_main;
     std::vector<int> idx;  // plan problem -> field
     std::vector<double> omegas(nf);
     for (uint32_t i = 0; i < nf; i++) {
@@ -695,9 +748,10 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     for (uint32_t i = 0; i < nf; i++)
         if ((F[i]->D > 1) != three_d)
             return fail(CW_ERR_ARG, "a batch mixes planes and volumes");
-    int shape = o.shape >= 0 ? o.shape : default_shape(o.s ? o.s : 4);
+    int shape = o.shape >= 0 ? o.shape : default_smooth_shape(o.s ? o.s : 8);
+    if (shape >= kNumSmoothShapes) return fail(CW_ERR_ARG, "unknown tile shape %d", shape);
     if (three_d) shape = 0;
-    const int s = kShapes[shape].s;
+    const int s = kSmoothShapes[shape].s;
     Plan outer, inner;
     outer.stream = inner.stream = stream;
     outer.fp64 = true;

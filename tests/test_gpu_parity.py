@@ -37,15 +37,29 @@ def test_bathed_disc_bitwise(size, radius, uptake, tol):
     assert bits_equal(g.download(), ref.values)
 
 
-@pytest.mark.parametrize("spp", [1, 2])
-def test_config1_max_change_bitwise(spp):
-    say(f"case config1 omega=1.9 spp={spp}")
+PLAIN_SHAPES = range(7)  # FP64 plain sweep kernels (LDS tiles and register-blocked)
+
+
+@pytest.mark.parametrize("shape", PLAIN_SHAPES)
+def test_config1_max_change_bitwise(shape):
+    say(f"case config1 omega=1.9 plain shape={shape}")
     (fc,) = cases.config1()
     ref = oracle_from_case(fc)
     g = gpu_from_case(fc)
     want = ref.relax(1e-10, 100_000, omega=1.9)
     st = g.relax_ex(omega=1.9, criterion=CRIT_MAX_CHANGE, tolerance=1e-10, max_sweeps=100_000,
-                    sweeps_per_pass=spp)
+                    tile_shape=shape)
+    assert st["sweeps"] == want
+    assert bits_equal(g.download(), ref.values)
+
+
+@pytest.mark.parametrize("shape", PLAIN_SHAPES)
+def test_general_form_every_shape(shape):
+    say(f"case general form plain shape={shape}")
+    ref, g = general_case()
+    want = ref.relax(1e-12, 100_000)
+    st = g.relax_ex(criterion=CRIT_MAX_CHANGE, tolerance=1e-12, max_sweeps=100_000,
+                    tile_shape=shape)
     assert st["sweeps"] == want
     assert bits_equal(g.download(), ref.values)
 
@@ -59,14 +73,15 @@ def test_config1_reference_dropin():
     assert bits_equal(g.download(), ref.values)
 
 
-def test_config1_rel_residual_bitwise():
-    say("case config1 rel residual 1e-8 spp=1 check every sweep")
+@pytest.mark.parametrize("shape", [0, 2, 6])
+def test_config1_rel_residual_bitwise(shape):
+    say(f"case config1 rel residual 1e-8 one sweep per pass, shape={shape}")
     (fc,) = cases.config1()
     ref = oracle_from_case(fc)
     g = gpu_from_case(fc)
     want = ref.relax_residual(1e-8, 100_000, omega=1.9)
     st = g.relax_ex(omega=1.9, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, max_sweeps=100_000,
-                    sweeps_per_pass=1, check_every=1)
+                    tile_shape=shape, check_every=1)
     assert st["sweeps"] == want
     assert bits_equal(g.download(), ref.values)
     rel = g.residual()
@@ -74,10 +89,9 @@ def test_config1_rel_residual_bitwise():
     assert abs(rel - ref.rel_residual()[0]) <= 1e-12 * max(1.0, rel) + 1e-15
 
 
-def test_general_form_bitwise():
+def general_case():
     """A uniform source and voxels that both take up and secrete: the
     unfolded ((d*sum + source) + p) - u form."""
-    say("case general form 40x40 source")
     n = 40
     rng = np.random.default_rng(7)
     ref = OracleField(n, n, 1, 0.3, 0.01, 0.5)
@@ -92,9 +106,7 @@ def test_general_form_bitwise():
     g.set_source(2e-3)
     g.upload(values=ref.values.copy(), held=ref.held.copy(), uptake=ref.uptake.copy(),
              production=ref.production.copy())
-    want = ref.relax(1e-12, 100_000)
-    assert g.relax(1e-12, 100_000) == want
-    assert bits_equal(g.download(), ref.values)
+    return ref, g
 
 
 def test_starving_ball_3d_bitwise():
@@ -168,14 +180,18 @@ def test_batch_equals_single():
         assert bits_equal(g.download(), v)
 
 
-def test_fp32_refine_reaches_residual_config1():
-    say("case config1 fp32 refine")
+SMOOTH_SHAPES = range(18)  # FP32 smoother kernels
+
+
+@pytest.mark.parametrize("shape", SMOOTH_SHAPES)
+def test_fp32_refine_reaches_residual_config1(shape):
+    say(f"case config1 fp32 refine shape={shape}")
     (fc,) = cases.config1()
     ref = oracle_from_case(fc)
     ref.relax_residual(1e-12, 200_000, omega=1.9)
     g = gpu_from_case(fc)
     st = g.relax_ex(omega=1.9, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, max_sweeps=200_000,
-                    precision=PREC_FP32_REFINE)
+                    precision=PREC_FP32_REFINE, tile_shape=shape)
     say(f"  refine stats {st}")
     v = g.download()
     chk = oracle_from_case(fc)
