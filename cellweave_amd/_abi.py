@@ -51,6 +51,22 @@ class RelaxStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class AdjointResult(C.Structure):
+    _fields_ = [
+        ("loss", C.c_double),
+        ("dl_dd", C.c_double),
+        ("dl_dg", C.c_double),
+        ("sweeps", C.c_uint64),
+        ("rel_residual", C.c_double),
+        ("seconds", C.c_double),
+        ("status", C.c_int32),
+        ("pad_", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad_"}
+
+
 def _declare():
     P = C.c_void_p
     st = C.c_int
@@ -78,8 +94,8 @@ def _declare():
         "cw_batch_relax": (st, [C.POINTER(P), u32, opts, stats]),
         "cw_field_fill": (st, [P, d]),
         "cw_batch_time_passes": (st, [C.POINTER(P), u32, opts, u32, dp]),
+        "cw_field_adjoint": (st, [P, dp, u32p, u32, opts, dp, C.POINTER(AdjointResult)]),
     })
-    del u32p
 
 
 _declare()

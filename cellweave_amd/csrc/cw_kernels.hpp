@@ -95,6 +95,8 @@ struct PassArgs {
     int crit, max_sweeps, mode, replay_step;
     double tol;
     const Ctl* inner_ctl;  // correct64: the inner plan's control blocks
+    int project;           // resid64: project onto the floor (0 for the adjoint's linear system)
+    int pad_;
 };
 
 template <typename T> struct V2;
@@ -1178,7 +1180,7 @@ __global__ __launch_bounds__(256) void resid64(PassArgs a, int write32) {
             const double b = kk2 ? ((pr.source + kq) - kk2[i]) : kq;
             const double r = (b + d * sum) - A * ci;
             double rt = r;
-            if (A > 0.0 && !(ci + r / A > 0.0)) rt = -A * ci;
+            if (a.project && A > 0.0 && !(ci + r / A > 0.0)) rt = -A * ci;
             const double be = b + d * hsum;
             r2 += rt * rt;
             b2 += be * be;
@@ -1268,6 +1270,93 @@ __global__ __launch_bounds__(256) void decide(PassArgs a) {
             }
         }
     }
+}
+
+// Adjoint set-up over the swept box: rhs = c - target and the smoother's
+// pre-scaled beta = omega * rhs / A on free cells (NaN on held ones), and
+// lambda = 0 in both buffers.
+__global__ __launch_bounds__(256) void adjoint_setup(PassArgs a, const double* c,
+                                                     const double* target, double* rhs,
+                                                     double* beta, double* lam0, double* lam1) {
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int p = a.tile_prob[t];
+    const Prob& pr = a.probs[p];
+    const ElTile et = el_tile(pr, t);
+    if (!et.ok) return;
+    const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
+    const size_t plane = (size_t)P * Hh;
+    const double* kk = (const double*)pr.k;
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const int x = et.gx + q;
+        if (x >= W) continue;
+        const size_t i = (size_t)et.gz * plane + (size_t)et.gy * P + x;
+        lam0[i] = 0.0;
+        lam1[i] = 0.0;
+        if (kk[i] != kk[i]) {
+            rhs[i] = nan_of<double>();
+            beta[i] = nan_of<double>();
+            continue;
+        }
+        const int real = real_count(x, et.gy, et.gz, W, Hh, D);
+        const double A = (double)real * pr.d + pr.decay;
+        const double r = c[i] - target[i];
+        rhs[i] = r;
+        beta[i] = pr.omega * r / A;
+    }
+}
+
+// Adjoint gradient partials per elementwise tile (r2 slot: dL/dD, b2 slot:
+// dL/dg from the free cells), for the closed forms
+//   dL/dD = sum_i lambda_i sum_{real nbrs j} (c_j - c_i),
+//   dL/dg = D sum_i lambda_i #held_nbrs(i)   (+ the held cells' own term).
+__global__ __launch_bounds__(256) void adjoint_grads(PassArgs a, const double* c,
+                                                     const double* lam, const double* kk) {
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int p = a.tile_prob[t];
+    const Prob& pr = a.probs[p];
+    double gd = 0.0, gg = 0.0;
+    const ElTile et = el_tile(pr, t);
+    if (et.ok) {
+        const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
+        const size_t plane = (size_t)P * Hh;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int x = et.gx + q;
+            if (x >= W) continue;
+            const size_t i = (size_t)et.gz * plane + (size_t)et.gy * P + x;
+            if (kk[i] != kk[i]) continue;
+            const double ci = c[i];
+            double ds = 0.0, nh = 0.0;
+#define CW_NB(cond, j)                       \
+    if (cond) {                              \
+        ds += c[j] - ci;                     \
+        if (kk[j] != kk[j]) nh += 1.0;       \
+    }
+            CW_NB(x > 0, i - 1)
+            CW_NB(x + 1 < W, i + 1)
+            CW_NB(et.gy > 0, i - P)
+            CW_NB(et.gy + 1 < Hh, i + P)
+            CW_NB(et.gz > 0, i - plane)
+            CW_NB(et.gz + 1 < D, i + plane)
+#undef CW_NB
+            gd += lam[i] * ds;
+            gg += pr.d * lam[i] * nh;
+        }
+    }
+    block_partial(a.partials, t, gd, gg);
+}
+
+// dL/du_k = -sum of lambda over the voxels whose uptake is cell k's rate,
+// one thread per cell over a CSR list, in a fixed order.
+__global__ __launch_bounds__(256) void cell_sums(const double* lam, const long long* off,
+                                                 const long long* idx, int ncells,
+                                                 double* out) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= ncells) return;
+    double s = 0.0;
+    for (long long j = off[k]; j < off[k + 1]; j++) s += lam[idx[j]];
+    out[k] = -s;
 }
 
 // Every cell of the box (not the row padding) set to v: ScalarField::fill

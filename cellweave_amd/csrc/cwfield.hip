@@ -322,6 +322,7 @@ PassArgs make_args(Plan& pl, int tl, int mode, int crit, double tol, int max_swe
     a.mode = mode;
     a.max_sweeps = max_sweeps;
     a.tol = tol;
+    a.project = 1;
     return a;
 }
 
@@ -500,10 +501,11 @@ cw_status run_passes(Plan& pl, int crit, double tol, uint64_t budget, uint64_t d
 
 // FP64 residual of every running problem (optionally writing the FP32
 // defect inputs), then r2/b2 recorded and converged problems stopped.
-cw_status run_resid(Plan& pl, int crit, double tol, int write32) {
+cw_status run_resid(Plan& pl, int crit, double tol, int write32, int project = 1) {
     const int nt = (int)pl.tp[TL_EL].size();
     if (nt == 0) return CW_OK;
     PassArgs a = make_args(pl, TL_EL, DECIDE_RESID, crit, tol, 0);
+    a.project = project;
     hipLaunchKernelGGL(resid64, dim3(nt), dim3(256), 0, pl.stream, a, write32);
     CW_HIP(hipGetLastError());
     cw_status st = launch_decide(pl, DECIDE_RESID, crit, tol, 0);
@@ -916,6 +918,216 @@ cw_status solve(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts, cw_r
     return CW_OK;
 }
 
+// Adjoint of the steady state (new; the reference fits parameters by hand).
+// Loss L = 0.5 * sum over all voxels of (c - target)^2. With A c = b on the
+// free voxels and lambda = A^-1 (c - target) (A symmetric, lambda = 0 on held
+// voxels), the gradients are
+//   dL/dD   = sum_i lambda_i sum_{real nbrs j} (c_j - c_i)   (D in px^2 per s)
+//   dL/du_k = -sum of lambda over the voxels taking up at cell k's rate
+//   dL/dg   = D sum_i lambda_i #held_nbrs(i) + sum_held (c_h - target_h)
+// valid while no free voxel is floored. lambda is solved in FP64 by the
+// pre-scaled red-black smoother, tested on its unprojected relative residual.
+cw_status field_adjoint(cw_field* f, const double* target, const uint32_t* grid,
+                        uint32_t ncells, const cw_relax_opts* opts, double* dl_du,
+                        cw_adjoint_result* res) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!f || !target || !res || !opts) return fail(CW_ERR_ARG, "null argument");
+    if (ncells && dl_du && !grid) return fail(CW_ERR_ARG, "per-cell gradients need the lattice");
+    Opts o;
+    cw_status st = check_opts(opts, &o);
+    if (st) return st;
+    if (f->coef_dirty && (st = build_coefficients(f))) return st;
+    memset(res, 0, sizeof *res);
+    res->rel_residual = std::nan("");
+    if (dl_du)
+        for (uint32_t k = 0; k < ncells; k++) dl_du[k] = 0.0;
+    // forward state on the host: loss, the held voxels' own term, the floor
+    std::vector<double> c(f->n);
+    CW_HIP(hipMemcpy2DAsync(c.data(), f->W * sizeof(double), f->c[f->cur], f->P * sizeof(double),
+                            f->W * sizeof(double), (size_t)f->H * f->D, hipMemcpyDeviceToHost,
+                            f->stream));
+    CW_HIP(hipStreamSynchronize(f->stream));
+    double loss = 0.0, dg_held = 0.0;
+    size_t floored = 0;
+    for (size_t i = 0; i < f->n; i++) {
+        const double r = c[i] - target[i];
+        loss += r * r;
+        if (f->held[i]) dg_held += r;
+        else if (!(c[i] > 0.0)) floored++;
+    }
+    res->loss = 0.5 * loss;
+    res->dl_dg = dg_held;
+    if (floored)
+        return fail(CW_ERR_ARG,
+                    "the floor is active at %zu free voxels: the linear adjoint does not apply",
+                    floored);
+    if (!f->has_span) {
+        res->status = CW_OK;
+        return CW_OK;
+    }
+    const size_t bytes = f->npad * sizeof(double);
+    double *tgt = nullptr, *rhs = nullptr, *beta = nullptr, *lam[2] = {nullptr, nullptr};
+    auto cleanup = [&]() {
+        (void)hipFree(tgt); (void)hipFree(rhs); (void)hipFree(beta);
+        (void)hipFree(lam[0]); (void)hipFree(lam[1]);
+    };
+    auto bail = [&](cw_status s2) { cleanup(); return s2; };
+    if (hipMalloc((void**)&tgt, bytes) != hipSuccess || hipMalloc((void**)&rhs, bytes) != hipSuccess ||
+        hipMalloc((void**)&beta, bytes) != hipSuccess ||
+        hipMalloc((void**)&lam[0], bytes) != hipSuccess ||
+        hipMalloc((void**)&lam[1], bytes) != hipSuccess)
+        return bail(fail(CW_ERR_DEVICE, "adjoint buffers: out of device memory"));
+    // Outside the swept box every voxel is held: NaN coefficients, zero lambda.
+    if (hipMemsetAsync(rhs, 0xFF, bytes, f->stream) != hipSuccess ||
+        hipMemsetAsync(beta, 0xFF, bytes, f->stream) != hipSuccess ||
+        hipMemsetAsync(lam[0], 0, bytes, f->stream) != hipSuccess ||
+        hipMemsetAsync(lam[1], 0, bytes, f->stream) != hipSuccess ||
+        hipMemsetAsync(tgt, 0, bytes, f->stream) != hipSuccess ||
+        hipMemcpy2DAsync(tgt, f->P * sizeof(double), target, f->W * sizeof(double),
+                         f->W * sizeof(double), (size_t)f->H * f->D, hipMemcpyHostToDevice,
+                         f->stream) != hipSuccess)
+        return bail(fail(CW_ERR_DEVICE, "adjoint target upload failed"));
+    const bool three_d = f->D > 1;
+    const double omega = o.omega > 0.0 ? o.omega : reference_omega(f);
+    // smoother plan (lambda, pre-scaled beta) and residual plan (lambda, rhs)
+    Plan sm, rs;
+    sm.stream = rs.stream = f->stream;
+    sm.fp64 = rs.fp64 = true;
+    sm.plain = false;
+    sm.three_d = rs.three_d = three_d;
+    sm.s_main = three_d ? 1 : 2;
+    sm.shape = three_d ? 0 : 1;
+    rs.s_main = 1;
+    Prob pr;
+    fill_prob_common(pr, f, omega);
+    pr.k = f->k;
+    pr.c64 = f->c[f->cur];
+    Ctl c0;
+    init_ctl(c0, ST_RUNNING);
+    sm.hp.push_back(pr); sm.hc.push_back(c0); sm.nprob = 1;
+    rs.hp.push_back(pr); rs.hc.push_back(c0); rs.nprob = 1;
+    if ((st = plan_upload(sm)) || (st = plan_upload(rs))) return bail(st);
+    {
+        const int nt = (int)sm.tp[TL_EL].size();
+        PassArgs a = make_args(sm, TL_EL, MODE_SWEEP, CRIT_NONE, 0.0, 0);
+        hipLaunchKernelGGL(adjoint_setup, dim3(nt), dim3(256), 0, f->stream, a,
+                           (const double*)f->c[f->cur], (const double*)tgt, rhs, beta, lam[0],
+                           lam[1]);
+        if (hipGetLastError() != hipSuccess) return bail(fail(CW_ERR_DEVICE, "adjoint_setup"));
+    }
+    int cur = 0;
+    auto set_state = [&]() -> cw_status {
+        sm.hp[0].x[0] = lam[cur];
+        sm.hp[0].x[1] = three_d ? lam[cur] : lam[cur ^ 1];
+        sm.hp[0].k = beta;
+        rs.hp[0].k = rhs;
+        rs.hp[0].c64 = lam[cur];
+        CW_HIP(hipMemcpyAsync(sm.dp, sm.hp.data(), sizeof(Prob), hipMemcpyHostToDevice, f->stream));
+        CW_HIP(hipMemcpyAsync(rs.dp, rs.hp.data(), sizeof(Prob), hipMemcpyHostToDevice, f->stream));
+        return CW_OK;
+    };
+    if ((st = set_state())) return bail(st);
+    int longest = 2;
+    for (int ax = 0; ax < 3; ax++) longest = std::max(longest, f->span[2 * ax + 1] - f->span[2 * ax] + 1);
+    std::vector<double> hist;
+    std::vector<int> at;
+    uint64_t total = 0;
+    bool converged = false;
+    while (true) {
+        run_reset(rs);
+        init_ctl(rs.hc[0], ST_RUNNING);
+        if ((st = write_ctl(rs)) || (st = run_resid(rs, CRIT_RESID, o.tol, 0, 0))) return bail(st);
+        const Ctl& rc = rs.hc[0];
+        const double rel = rc.b2 > 0.0 ? std::sqrt(rc.r2 / rc.b2) : std::sqrt(rc.r2);
+        res->rel_residual = rel;
+        if (rc.nonfinite) return bail(fail(CW_ERR_NONFINITE, "the adjoint residual is not finite"));
+        if (rc.status == ST_DONE) { converged = true; break; }
+        if (total >= o.max_sweeps) break;
+        hist.push_back(rel);
+        at.push_back((int)total);
+        int k = next_segment(hist, at, o.tol, 1e-300, std::max(32, longest / 2), 16,
+                             std::max(256, 8 * longest), sm.s_main);
+        k = (int)std::min<uint64_t>((uint64_t)k, o.max_sweeps - total);
+        init_ctl(sm.hc[0], ST_RUNNING);
+        sm.hc[0].budget = k;
+        if ((st = write_ctl(sm))) return bail(st);
+        run_reset(sm);
+        LoopResult lr;
+        if ((st = run_passes(sm, CRIT_NONE, 0.0, (uint64_t)k, 0x7fffffff, 1, &lr))) return bail(st);
+        total += (uint64_t)sm.hc[0].sweeps;
+        if (!three_d && (sm.hc[0].parity & 1)) cur ^= 1;
+        if ((st = set_state())) return bail(st);
+    }
+    res->sweeps = total;
+    // gradient partials over the swept box
+    {
+        const int nt = (int)rs.tp[TL_EL].size();
+        PassArgs a = make_args(rs, TL_EL, MODE_SWEEP, CRIT_NONE, 0.0, 0);
+        hipLaunchKernelGGL(adjoint_grads, dim3(nt), dim3(256), 0, f->stream, a,
+                           (const double*)f->c[f->cur], (const double*)lam[cur],
+                           (const double*)f->k);
+        if (hipGetLastError() != hipSuccess) return bail(fail(CW_ERR_DEVICE, "adjoint_grads"));
+        std::vector<double> part(2 * (size_t)nt);
+        CW_HIP(hipMemcpyAsync(part.data(), rs.partials, part.size() * sizeof(double),
+                              hipMemcpyDeviceToHost, f->stream));
+        CW_HIP(hipStreamSynchronize(f->stream));
+        double gd = 0.0, gg = 0.0;
+        for (int t = 0; t < nt; t++) { gd += part[2 * t]; gg += part[2 * t + 1]; }
+        res->dl_dd = gd;
+        res->dl_dg += gg;
+    }
+    if (dl_du && ncells) {
+        // CSR of cell -> voxels taking up at that cell's rate (pitched indices)
+        std::vector<long long> off(ncells + 1, 0);
+        const size_t W = f->W, H = f->H;
+        for (size_t i = 0; i < f->n; i++) {
+            const uint32_t id = grid[i];
+            if (f->held[i] || !(f->uptake[i] > 0.0) || id == 0 || id == 0xFFFFFFFFu || id >= ncells)
+                continue;
+            off[id + 1]++;
+        }
+        for (uint32_t k = 0; k < ncells; k++) off[k + 1] += off[k];
+        std::vector<long long> idx((size_t)off[ncells]);
+        std::vector<long long> fillp(off.begin(), off.end() - 1);
+        for (size_t i = 0; i < f->n; i++) {
+            const uint32_t id = grid[i];
+            if (f->held[i] || !(f->uptake[i] > 0.0) || id == 0 || id == 0xFFFFFFFFu || id >= ncells)
+                continue;
+            const size_t x = i % W, y = (i / W) % H, z = i / (W * H);
+            idx[(size_t)fillp[id]++] = (long long)((z * H + y) * (size_t)f->P + x);
+        }
+        long long *doff = nullptr, *didx = nullptr;
+        double* dout = nullptr;
+        bool ok = hipMalloc((void**)&doff, off.size() * sizeof(long long)) == hipSuccess &&
+                  hipMalloc((void**)&didx, std::max<size_t>(1, idx.size()) * sizeof(long long)) == hipSuccess &&
+                  hipMalloc((void**)&dout, ncells * sizeof(double)) == hipSuccess;
+        if (ok) {
+            ok = hipMemcpyAsync(doff, off.data(), off.size() * sizeof(long long),
+                                hipMemcpyHostToDevice, f->stream) == hipSuccess &&
+                 (idx.empty() || hipMemcpyAsync(didx, idx.data(), idx.size() * sizeof(long long),
+                                                hipMemcpyHostToDevice, f->stream) == hipSuccess);
+            if (ok) {
+                hipLaunchKernelGGL(cell_sums, dim3((ncells + 255) / 256), dim3(256), 0, f->stream,
+                                   (const double*)lam[cur], (const long long*)doff,
+                                   (const long long*)didx, (int)ncells, dout);
+                ok = hipGetLastError() == hipSuccess &&
+                     hipMemcpyAsync(dl_du, dout, ncells * sizeof(double), hipMemcpyDeviceToHost,
+                                    f->stream) == hipSuccess &&
+                     hipStreamSynchronize(f->stream) == hipSuccess;
+            }
+        }
+        (void)hipFree(doff); (void)hipFree(didx); (void)hipFree(dout);
+        if (!ok) return bail(fail(CW_ERR_DEVICE, "per-cell gradient sums failed"));
+    }
+    cleanup();
+    res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    res->status = converged ? CW_OK : CW_ERR_NOT_CONVERGED;
+    if (!converged)
+        return fail(CW_ERR_NOT_CONVERGED, "the adjoint did not reach %g within %llu sweeps",
+                    o.tol, (unsigned long long)o.max_sweeps);
+    return CW_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1080,6 +1292,12 @@ cw_status cw_field_fill(cw_field* f, double value) {
     CW_HIP(hipGetLastError());
     CW_HIP(hipStreamSynchronize(f->stream));
     return CW_OK;
+}
+
+cw_status cw_field_adjoint(cw_field* f, const double* target, const uint32_t* grid,
+                           uint32_t ncells, const cw_relax_opts* opts, double* dl_du,
+                           cw_adjoint_result* result) {
+    return field_adjoint(f, target, grid, ncells, opts, dl_du, result);
 }
 
 cw_status cw_field_residual(cw_field* f, double* rel) {

@@ -133,6 +133,27 @@ class Field:
         _check(_abi.lib().cw_field_relax_ex(self._h, C.byref(o), C.byref(s)))
         return s.as_dict()
 
+    def adjoint(self, target, grid=None, ncells=0, tolerance=1e-12, omega=0.0,
+                max_sweeps=2_000_000) -> dict:
+        """Gradients of 0.5*sum((c - target)^2) with respect to D, the held
+        value and the per-cell uptake rates (cw_field_adjoint)."""
+        t = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
+        assert t.size == self.n
+        gp = None
+        if grid is not None:
+            g = np.ascontiguousarray(grid, dtype=np.uint32).reshape(-1)
+            assert g.size == self.n
+            gp = g.ctypes.data_as(C.POINTER(C.c_uint32))
+        du = np.zeros(max(int(ncells), 1), dtype=np.float64)
+        o = make_opts(omega=omega, criterion=CRIT_REL_RESIDUAL, tolerance=tolerance,
+                      max_sweeps=max_sweeps)
+        r = _abi.AdjointResult()
+        _check(_abi.lib().cw_field_adjoint(self._h, _dptr(t), gp, int(ncells), C.byref(o),
+                                           _dptr(du), C.byref(r)))
+        out = r.as_dict()
+        out["du"] = du[:ncells]
+        return out
+
     def residual(self) -> float:
         r = C.c_double()
         _check(_abi.lib().cw_field_residual(self._h, C.byref(r)))

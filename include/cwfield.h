@@ -139,6 +139,32 @@ cw_status cw_field_residual(cw_field *field, double *rel_residual);
 cw_status cw_batch_relax(cw_field *const *fields, uint32_t count, const cw_relax_opts *opts,
                          cw_relax_stats *stats);
 
+/* Result of cw_field_adjoint. */
+typedef struct cw_adjoint_result {
+    double loss;          /* 0.5 * sum over all voxels of (c - target)^2 */
+    double dl_dd;         /* dL/dD, D in px^2 per unit time (divide by dx^2 for um^2/s) */
+    double dl_dg;         /* dL/d(value of every held voxel) */
+    uint64_t sweeps;      /* adjoint SOR sweeps */
+    double rel_residual;  /* ||(c - target) - A lambda|| / ||c - target|| at exit */
+    double seconds;
+    int32_t status;
+    int32_t pad_;
+} cw_adjoint_result;
+
+/* Adjoint gradients of the steady state (new: cellweave fits parameters by
+ * hand). The device values must be the solved field (after a relax). Solves
+ * A lambda = (c - target) on the free voxels in FP64 to opts->tolerance
+ * (relative residual; opts->omega, opts->max_sweeps apply) and returns
+ *   dL/dD = sum_i lambda_i sum_{real nbrs j} (c_j - c_i),
+ *   dL/dg = D sum_i lambda_i #held_nbrs(i) + sum_held (c_h - target_h),
+ *   dl_du[k] = dL/d(uptake rate of cell k) = -sum of lambda over the free
+ *              voxels of cell k (grid ids, x fastest) whose uptake is > 0,
+ * for k < ncells (dl_du and grid may be NULL). CW_ERR_ARG if a free voxel
+ * sits on the floor (the gradient of the floored problem is not linear). */
+cw_status cw_field_adjoint(cw_field *field, const double *target, const uint32_t *grid,
+                           uint32_t ncells, const cw_relax_opts *opts, double *dl_du,
+                           cw_adjoint_result *result);
+
 /* Measurement helper (no reference counterpart): builds the same launch plan
  * cw_batch_relax would for `opts` and times `passes` back-to-back launches of
  * its sweep kernel with HIP events on the library's stream. *ms_per_pass gets
