@@ -53,7 +53,7 @@ struct Prob {
     int sw_base[2], sw_count[2], ntx[2], nty[2];  // sweep tilings: [0] s sweeps, [1] one
     int el_base, el_count, elx, ely;              // elementwise tiling (128 x 4 per plane)
     int inner_idx;           // this problem's index in the inner plan (refinement)
-    int pad_;
+    int zc;                  // 3D streaming: planes per z chunk
     void* x[2];              // state ping-pong (T); equal for in-place 3D sweeps
     const void* k;           // coefficient (T): NaN = held. FP64: production minus
                              // uptake when at most one is nonzero; else production
@@ -1037,6 +1037,241 @@ __global__ __launch_bounds__(NW * 64) void relax2d_reg(PassArgs a) {
         if (a.check) block_partial(a.partials + (a.pass & 1) * a.part_stride, t, r2, 0.0);
         if (a.crit == CRIT_MAXCHANGE && a.mode == MODE_SWEEP)
             block_moved<NW, (S > kMaxS ? 1 : S)>(a.ctl[p], a.pass, mv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3D red-black sweep streaming in z (one full sweep per launch, ping-pong).
+//
+// A workgroup owns a 128 x (NW*NR) tile in (x, y) (halo 2, interior
+// 124 x (NW*NR - 4)) and a chunk of zc planes. Lanes hold column pairs, waves
+// contiguous rows, as in relax2d_reg, and each plane's cells sit in registers
+// by colour. Colour flips with z, so the z neighbours of st[c][i] in plane z
+// are st[1-c][i] of planes z-1 and z+1: the same register slot. Step z loads
+// plane z+1, updates red in plane z (old black around it), then black in plane
+// z-1 (new red around it) and stores plane z-1: every plane is read and
+// written once per sweep. The chunk recomputes red one plane before its first
+// plane; in-plane halos are recomputed as in the 2D tiles.
+// ---------------------------------------------------------------------------
+template <typename T, int NR>
+struct Plane {
+    T st[2][NR];
+};
+template <typename T, int NR>
+struct PlaneK {
+    T k[2][NR];   // coefficient (NaN = held or outside); PLAIN: folded k or production
+    T u[2][NR];   // PLAIN+GEN: uptake; smoother: lower bound (-inf without one)
+};
+
+template <typename T, bool PLAIN, bool GEN, int NR, int C, int P0>
+__device__ __forceinline__ void plane_update(Plane<T, NR>& cur, const Plane<T, NR>& below_z,
+                                             const Plane<T, NR>& above_z, const PlaneK<T, NR>& K,
+                                             T above, T below, int lane, const int (&rx)[2],
+                                             const int (&ry)[NR], int rz, T d, T decay, T omega,
+                                             T omo, T source, bool track, int r0, int TYlo,
+                                             int TYhi, unsigned long long* mv) {
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int q = P0 ^ (i & 1);
+        const T o = cur.st[C ^ 1][i];
+        const T across = q ? from_right(o) : from_left(o);
+        const T up = i > 0 ? cur.st[C ^ 1][i - 1] : above;
+        const T dn = i < NR - 1 ? cur.st[C ^ 1][i + 1] : below;
+        const T zm = below_z.st[C ^ 1][i];
+        const T zp = above_z.st[C ^ 1][i];
+        const T kq = K.k[C][i];
+        const T before = cur.st[C][i];
+        const int real = rx[q] + ry[i] + rz;
+        T after;
+        if (PLAIN) {
+            // x-1, x+1, y-1, y+1, z-1, z+1, in the reference's order
+            T sum = T(0);
+            if (q == 0) { sum += across; sum += o; } else { sum += o; sum += across; }
+            sum += up;
+            sum += dn;
+            sum += zm;
+            sum += zp;
+            T num = d * sum;
+            if (GEN) {
+                num = num + source;
+                num = num + kq;
+                num = num - K.u[C][i];
+            } else {
+                num = num + kq;
+            }
+            const T bal = num / ((T)real * d + decay);
+            after = omo * before + omega * bal;
+            after = after > T(0) ? after : T(0);
+        } else {
+            const T sum = ((across + o) + (up + dn)) + (zm + zp);
+            const T a = omega * d * rcp_t((T)real * d + decay);
+            after = fma_t(omo, before, fma_t(a, sum, kq));
+            after = max_t(after, K.u[C][i]);
+        }
+        const bool upd = kq == kq;
+        if (PLAIN && track && upd) {
+            const int x = 2 * lane + q;
+            const int r = r0 + i;
+            if (x >= 2 && x < 126 && r >= TYlo && r < TYhi) {
+                const unsigned long long m = change_bits((double)__builtin_fabs(after - before));
+                *mv = m > *mv ? m : *mv;
+            }
+        }
+        cur.st[C][i] = upd ? after : before;
+    }
+}
+
+template <typename T, int NW, int NR, bool PLAIN, bool GEN>
+__global__ __launch_bounds__(NW * 64) void relax3d_stream(PassArgs a) {
+    constexpr int RY = NW * NR;
+    constexpr int TX = 124, TY = RY - 4;
+    static_assert(NR >= 2 && TY > 0, "tile shape");
+    using T2 = typename V2<T>::type;
+    __shared__ T edge[2][NW][2][64];  // [slot][wave][first/last row][lane]: one colour
+
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int p = a.tile_prob[t];
+    const Prob& pr = a.probs[p];
+    const int tl = a.tl;
+    const int lt = t - pr.sw_base[tl];
+    const int nxy = pr.ntx[tl] * pr.nty[tl];
+    const int ch = lt / nxy, txy = lt % nxy;
+    const int ti = txy % pr.ntx[tl], tj = txy / pr.ntx[tl];
+    const int gxo = pr.xs + ti * TX - 2;
+    const int gyo = pr.ys + tj * TY - 2;
+    const int za = pr.zs + ch * pr.zc;
+    const int zb = za + pr.zc - 1 < pr.z1 ? za + pr.zc - 1 : pr.z1;
+    const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
+    const size_t plane = (size_t)P * Hh;
+    const int gx = gxo + 2 * lane;
+    const bool colin = gx >= 0 && gx < W;
+    const T* kk = (const T*)pr.k;
+    const T* k2p = (const T*)(PLAIN ? pr.k2 : pr.lb);
+    const bool has_k2 = PLAIN ? GEN : (k2p != nullptr);
+    const T d = (T)pr.d, decay = (T)pr.decay, omega = (T)pr.omega, omo = (T)pr.omo;
+    const T source = (T)pr.source;
+    const int r0 = wv * NR;
+    const int rx[2] = {(gx > 0) + (gx < W - 1), (gx + 1 > 0) + (gx + 1 < W - 1)};
+    int ry[NR];
+    bool rowin[NR];
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int gy = gyo + r0 + i;
+        ry[i] = (gy > 0) + (gy < Hh - 1);
+        rowin[i] = colin && gy >= 0 && gy < Hh;
+    }
+    const int base0 = (gxo + gyo + r0) & 1;  // colour of column 2l, row 0, plane 0
+    const T* src = (const T*)pr.x[a.pass & 1];
+
+    auto load_state = [&](Plane<T, NR>& pl, int z) {
+        const int sw0 = (base0 + z) & 1;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            T ve = T(0), vo = T(0);
+            if (rowin[i] && z >= 0 && z < D) {
+                const T2 v = *(const T2*)(src + (size_t)z * plane + (size_t)(gyo + r0 + i) * P + gx);
+                ve = v.x; vo = v.y;
+            }
+            const bool sw = (sw0 ^ (i & 1)) != 0;
+            pl.st[0][i] = sw ? vo : ve;
+            pl.st[1][i] = sw ? ve : vo;
+        }
+    };
+    auto load_coef = [&](PlaneK<T, NR>& K, int z) {
+        const int sw0 = (base0 + z) & 1;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            T ce = nan_of<T>(), co = nan_of<T>(), ue = PLAIN ? T(0) : neg_inf<T>(), uo = ue;
+            if (rowin[i] && z >= 0 && z < D) {
+                const size_t off = (size_t)z * plane + (size_t)(gyo + r0 + i) * P + gx;
+                const T2 c = *(const T2*)(kk + off);
+                ce = c.x; co = c.y;
+                if (has_k2) { const T2 u = *(const T2*)(k2p + off); ue = u.x; uo = u.y; }
+            }
+            const bool sw = (sw0 ^ (i & 1)) != 0;
+            K.k[0][i] = sw ? co : ce; K.k[1][i] = sw ? ce : co;
+            K.u[0][i] = sw ? uo : ue; K.u[1][i] = sw ? ue : uo;
+        }
+    };
+
+    Plane<T, NR> m2, m1, c0, p1;
+    PlaneK<T, NR> km1, k0, kp1;
+    // pipeline fill: planes za-2, za-1, za; coefficients za-1, za
+    load_state(m2, za - 2);
+    load_state(m1, za - 1);
+    load_state(c0, za);
+    load_coef(km1, za - 1);
+    load_coef(k0, za);
+
+    int par;
+    if (!sweep_prologue(a, pr, p, lt, &par)) return;
+    T* dst = (T*)pr.x[par ^ 1];
+
+    unsigned long long mv = 0ull;
+    const bool track = PLAIN && a.crit == CRIT_MAXCHANGE;
+    int slot = 0;
+    // red in one plane: publish the plane's other-colour edge rows, then update
+    auto red_phase = [&](Plane<T, NR>& cur, const Plane<T, NR>& bz, const Plane<T, NR>& az,
+                         const PlaneK<T, NR>& K, int z, bool mtrack) {
+        edge[slot][wv][0][lane] = cur.st[1][0];
+        edge[slot][wv][1][lane] = cur.st[1][NR - 1];
+        __syncthreads();
+        const T above = wv > 0 ? edge[slot][wv - 1][1][lane] : T(0);
+        const T below = wv < NW - 1 ? edge[slot][wv + 1][0][lane] : T(0);
+        slot ^= 1;
+        const int rz = (z > 0) + (z < D - 1);
+        const int p0 = (base0 + z) & 1;  // x parity of colour 0 on row 0 of this plane
+        if (p0) plane_update<T, PLAIN, GEN, NR, 0, 1>(cur, bz, az, K, above, below, lane, rx, ry, rz, d, decay, omega, omo, source, mtrack, r0, 2, 2 + TY, &mv);
+        else plane_update<T, PLAIN, GEN, NR, 0, 0>(cur, bz, az, K, above, below, lane, rx, ry, rz, d, decay, omega, omo, source, mtrack, r0, 2, 2 + TY, &mv);
+    };
+    auto black_phase = [&](Plane<T, NR>& cur, const Plane<T, NR>& bz, const Plane<T, NR>& az,
+                           const PlaneK<T, NR>& K, int z, bool mtrack) {
+        edge[slot][wv][0][lane] = cur.st[0][0];
+        edge[slot][wv][1][lane] = cur.st[0][NR - 1];
+        __syncthreads();
+        const T above = wv > 0 ? edge[slot][wv - 1][1][lane] : T(0);
+        const T below = wv < NW - 1 ? edge[slot][wv + 1][0][lane] : T(0);
+        slot ^= 1;
+        const int rz = (z > 0) + (z < D - 1);
+        const int p1_ = ((base0 + z) & 1) ^ 1;  // x parity of colour 1 on row 0
+        if (p1_) plane_update<T, PLAIN, GEN, NR, 1, 1>(cur, bz, az, K, above, below, lane, rx, ry, rz, d, decay, omega, omo, source, mtrack, r0, 2, 2 + TY, &mv);
+        else plane_update<T, PLAIN, GEN, NR, 1, 0>(cur, bz, az, K, above, below, lane, rx, ry, rz, d, decay, omega, omo, source, mtrack, r0, 2, 2 + TY, &mv);
+    };
+
+    // red of plane za-1 (needed by black of za), only inside the swept box
+    if (za - 1 >= pr.zs) red_phase(m1, m2, c0, km1, za - 1, false);
+    for (int z = za; z <= zb + 1; z++) {
+        load_state(p1, z + 1);
+        load_coef(kp1, z + 1);
+        if (z <= pr.z1) red_phase(c0, m1, p1, k0, z, track && z <= zb);
+        if (z - 1 >= za) {
+            black_phase(m1, m2, c0, km1, z - 1, track);
+            // store plane z-1: interior rows and pairs inside the swept box
+            if (lane >= 1 && lane < 63 && gx <= pr.x1) {
+                const int zz = z - 1;
+                const int sw0 = (base0 + zz) & 1;
+#pragma unroll
+                for (int i = 0; i < NR; i++) {
+                    const int r = r0 + i;
+                    if (r < 2 || r >= 2 + TY) continue;
+                    const int gy = gyo + r;
+                    if (gy > pr.y1) continue;
+                    const bool sw = (sw0 ^ (i & 1)) != 0;
+                    T2 v;
+                    v.x = sw ? m1.st[1][i] : m1.st[0][i];
+                    v.y = sw ? m1.st[0][i] : m1.st[1][i];
+                    *(T2*)(dst + (size_t)zz * plane + (size_t)gy * P + gx) = v;
+                }
+            }
+        }
+        m2 = m1; m1 = c0; c0 = p1;
+        km1 = k0; k0 = kp1;
+    }
+    if (track) {
+        unsigned long long mvv[1] = {mv};
+        block_moved<NW, 1>(a.ctl[p], a.pass, mvv);
     }
 }
 

@@ -84,6 +84,14 @@ constexpr Shape kSmoothShapes[] = {{1, 32, 4, 0},   {2, 32, 4, 0},   {4, 64, 8, 
                                    {2, 64, 8, 2},   {4, 128, 16, 2}};
 constexpr int kNumPlainShapes = (int)(sizeof(kPlainShapes) / sizeof(kPlainShapes[0]));
 constexpr int kNumSmoothShapes = (int)(sizeof(kSmoothShapes) / sizeof(kSmoothShapes[0]));
+// 3D shapes: kind 0 = in-place red and black half sweeps (half3d, two
+// launches per sweep); kind 1 = z-streaming relax3d_stream with NW waves of
+// NR rows and zc planes per chunk (one launch per sweep, ping-pong).
+struct Shape3 { int kind, nw, nr, zc; };
+constexpr Shape3 k3DShapes[] = {{0, 4, 1, 1}, {1, 8, 4, 16}, {1, 4, 8, 16}, {1, 8, 4, 32}, {1, 8, 4, 8}};
+constexpr int kNum3DShapes = (int)(sizeof(k3DShapes) / sizeof(k3DShapes[0]));
+constexpr int kDefault3DShape = 1;
+
 int default_plain_shape(int s) { return s <= 1 ? 2 : 3; }
 int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 : 15; }
 
@@ -227,6 +235,8 @@ struct Plan {
     bool plain = true;   // FP64: floored field (false: the unfloored adjoint)
     bool general = false; // some problem keeps production and uptake apart
     bool three_d = false;
+    int shape3 = kDefault3DShape;   // k3DShapes index for volumes
+    bool in_place() const { return three_d && k3DShapes[shape3].kind == 0; }
     std::vector<Prob> hp;
     std::vector<Ctl> hc;
     std::vector<int> tp[3];
@@ -269,7 +279,14 @@ cw_status plan_upload(Plan& pl) {
             if (tl == TL_EL) {
                 nx = (sx + 127) / 128; ny = (sy + 3) / 4; nz = sz;
             } else if (pl.three_d) {
-                nx = ny = nz = 0;
+                const Shape3 sh = k3DShapes[pl.shape3];
+                if (sh.kind == 0) {
+                    nx = ny = nz = 0;
+                } else {
+                    const int ty = sh.nw * sh.nr - 4;
+                    nx = (sx + 123) / 124; ny = (sy + ty - 1) / ty; nz = (sz + sh.zc - 1) / sh.zc;
+                    pr.zc = sh.zc;
+                }
             } else {
                 const int si = tl == TL_MAIN ? pl.shape : 0;
                 const Shape sh = (pl.fp64 && pl.plain) ? kPlainShapes[si] : kSmoothShapes[si];
@@ -388,6 +405,33 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a) {
     return CW_OK;
 }
 
+cw_status launch_stream3d(Plan& pl, const PassArgs& a) {
+    const int nt = (int)pl.tp[a.tl].size();
+    if (nt == 0) return CW_OK;
+    const dim3 grid(nt), b256(256), b512(512);
+    hipStream_t st = pl.stream;
+    const int sh = pl.shape3;
+#define CW_L(kern, blk) hipLaunchKernelGGL(kern, grid, blk, 0, st, a)
+    if (pl.fp64 && pl.plain) {
+        if (pl.general) {
+            if (sh == 2) CW_L((relax3d_stream<double, 4, 8, true, true>), b256);
+            else CW_L((relax3d_stream<double, 8, 4, true, true>), b512);
+        } else {
+            if (sh == 2) CW_L((relax3d_stream<double, 4, 8, true, false>), b256);
+            else CW_L((relax3d_stream<double, 8, 4, true, false>), b512);
+        }
+    } else if (pl.fp64) {
+        if (sh == 2) CW_L((relax3d_stream<double, 4, 8, false, false>), b256);
+        else CW_L((relax3d_stream<double, 8, 4, false, false>), b512);
+    } else {
+        if (sh == 2) CW_L((relax3d_stream<float, 4, 8, false, false>), b256);
+        else CW_L((relax3d_stream<float, 8, 4, false, false>), b512);
+    }
+#undef CW_L
+    CW_HIP(hipGetLastError());
+    return CW_OK;
+}
+
 cw_status launch_half3d(Plan& pl, PassArgs a, int colour) {
     const int nt = (int)pl.tp[TL_EL].size();
     if (nt == 0) return CW_OK;
@@ -433,10 +477,17 @@ void run_reset(Plan& pl) {
 cw_status one_pass(Plan& pl, int s, int crit, int check, double tol, int max_sweeps) {
     cw_status st;
     if (pl.three_d) {
-        PassArgs a = make_args(pl, TL_EL, MODE_SWEEP, crit, tol, max_sweeps);
-        a.s = 1;
-        if ((st = launch_half3d(pl, a, 0))) return st;
-        if ((st = launch_half3d(pl, a, 1))) return st;
+        if (pl.in_place()) {
+            PassArgs a = make_args(pl, TL_EL, MODE_SWEEP, crit, tol, max_sweeps);
+            a.s = 1;
+            if ((st = launch_half3d(pl, a, 0))) return st;
+            if ((st = launch_half3d(pl, a, 1))) return st;
+        } else {
+            PassArgs a = make_args(pl, TL_MAIN, MODE_SWEEP, crit, tol, max_sweeps);
+            a.s = 1;
+            if ((st = launch_stream3d(pl, a))) return st;
+            pl.prev_tl = TL_MAIN;
+        }
         s = 1;
         check = 0;
     } else {
@@ -626,6 +677,12 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     pl.shape = o.shape >= 0 ? o.shape : default_plain_shape(s);
     if (pl.shape >= kNumPlainShapes) return fail(CW_ERR_ARG, "unknown tile shape %d", pl.shape);
     pl.s_main = kPlainShapes[pl.shape].s;
+    if (three_d) {
+        // volumes: tile_shape picks a 3D shape, one sweep per launch
+        pl.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape : kDefault3DShape;
+        pl.shape = 0;
+        pl.s_main = 1;
+    }
     s = pl.s_main;
     std::vector<int> idx;  // plan problem -> field
     std::vector<double> omegas(nf);
@@ -652,7 +709,7 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
         Prob pr;
         fill_prob_common(pr, f, omegas[i]);
         pr.x[0] = f->c[f->cur];
-        pr.x[1] = three_d ? f->c[f->cur] : f->c[f->cur ^ 1];
+        pr.x[1] = (three_d && k3DShapes[pl.shape3].kind == 0) ? f->c[f->cur] : f->c[f->cur ^ 1];
         pr.k = f->k;
         pr.k2 = f->general ? f->k2 : nullptr;
         if (f->general) pl.general = true;
@@ -671,7 +728,7 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     if (o.time_passes > 0) {
         if ((st = time_launches(pl, o.time_passes, o.time_ms))) return st;
         for (int p = 0; p < pl.nprob; p++)
-            if (!three_d && (pl.pass & 1)) F[idx[p]]->cur ^= 1;
+            if (!pl.in_place() && (pl.pass & 1)) F[idx[p]]->cur ^= 1;
         return CW_OK;
     }
     const int check_every = o.check_every ? o.check_every : (three_d ? 8 : 4);
@@ -686,6 +743,9 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
             const uint64_t n =
                 std::min<uint64_t>(check_every, o.max_sweeps - (uint64_t)pl.sweeps_before);
             if ((st = run_passes(pl, CRIT_NONE, o.tol, n, o.max_sweeps, 1, &lr))) return st;
+            for (int p = 0; p < pl.nprob; p++) pl.hp[p].c64 = (double*)pl.hp[p].x[pl.pass & 1];
+            CW_HIP(hipMemcpyAsync(pl.dp, pl.hp.data(), pl.nprob * sizeof(Prob),
+                                  hipMemcpyHostToDevice, pl.stream));
             if ((st = run_resid(pl, CRIT_RESID, o.tol, 0))) return st;
         }
         if ((st = launch_decide(pl, DECIDE_PASS, CRIT_NONE, o.tol,
@@ -699,7 +759,7 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     for (int p = 0; p < pl.nprob; p++) {
         cw_field* f = F[idx[p]];
         const Ctl& c = pl.hc[p];
-        if (!three_d && (c.parity & 1)) f->cur ^= 1;
+        if (!pl.in_place() && (c.parity & 1)) f->cur ^= 1;
         cw_status fs = CW_OK;
         if (c.nonfinite) fs = CW_ERR_NONFINITE;
         else if (c.status != ST_DONE) fs = CW_ERR_NOT_CONVERGED;
@@ -762,6 +822,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     outer.s_main = 1;
     inner.s_main = s;
     inner.shape = shape;
+    if (three_d) inner.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape : kDefault3DShape;
     std::vector<int> idx;
     std::vector<double> omegas(nf);
     int longest = 2;
@@ -789,7 +850,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         outer.hp.push_back(pr);
         Prob ip = pr;
         ip.x[0] = f->e[0];
-        ip.x[1] = three_d ? f->e[0] : f->e[1];
+        ip.x[1] = (three_d && k3DShapes[inner.shape3].kind == 0) ? f->e[0] : f->e[1];
         ip.k = f->r32;
         ip.k2 = nullptr;
         ip.lb = pr.lb32;
@@ -1018,7 +1079,7 @@ cw_status field_adjoint(cw_field* f, const double* target, const uint32_t* grid,
     int cur = 0;
     auto set_state = [&]() -> cw_status {
         sm.hp[0].x[0] = lam[cur];
-        sm.hp[0].x[1] = three_d ? lam[cur] : lam[cur ^ 1];
+        sm.hp[0].x[1] = sm.in_place() ? lam[cur] : lam[cur ^ 1];
         sm.hp[0].k = beta;
         rs.hp[0].k = rhs;
         rs.hp[0].c64 = lam[cur];
@@ -1055,7 +1116,7 @@ cw_status field_adjoint(cw_field* f, const double* target, const uint32_t* grid,
         LoopResult lr;
         if ((st = run_passes(sm, CRIT_NONE, 0.0, (uint64_t)k, 0x7fffffff, 1, &lr))) return bail(st);
         total += (uint64_t)sm.hc[0].sweeps;
-        if (!three_d && (sm.hc[0].parity & 1)) cur ^= 1;
+        if (!sm.in_place() && (sm.hc[0].parity & 1)) cur ^= 1;
         if ((st = set_state())) return bail(st);
     }
     res->sweeps = total;

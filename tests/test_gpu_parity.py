@@ -201,3 +201,81 @@ def test_fp32_refine_reaches_residual_config1(shape):
     err = np.max(np.abs(v - ref.values))
     assert err < 1e-6, err
     assert st["outer_iters"] >= 1
+
+
+SHAPES_3D = range(5)  # in-place half sweeps (0) and z-streaming variants (1-4)
+
+
+def ball_case(side=31, r=10.0, u=0.006, general=False):
+    c = side // 2
+    zz, yy, xx = np.meshgrid(np.arange(side), np.arange(side), np.arange(side), indexing="ij")
+    inside = ((xx - c) ** 2 + (yy - c) ** 2 + (zz - c) ** 2 <= r * r).reshape(-1)
+    ref = OracleField(side, side, side, D, 0.01 if general else 0.0, 1.0)
+    ref.uptake[:] = np.where(inside, u, 0.0)
+    if general:
+        ref.set_source(1e-4)
+        ref.production[:] = np.where(inside & (xx.reshape(-1) % 3 == 0), 2e-3, 0.0)
+    ref.hold_bath_at(1.0, (~inside).astype(np.uint8))
+    ref.copy_next_from_values()
+    g = Field(side, side, side, D, ref.decay, 1.0)
+    if general:
+        g.set_source(1e-4)
+    g.upload(values=ref.values.copy(), held=ref.held.copy(), uptake=ref.uptake.copy(),
+             production=ref.production.copy())
+    return ref, g
+
+
+@pytest.mark.parametrize("shape", SHAPES_3D)
+@pytest.mark.parametrize("general", [False, True])
+def test_ball_3d_every_shape_bitwise(shape, general):
+    say(f"case 3d ball shape={shape} general={general}")
+    ref, g = ball_case(general=general)
+    want = ref.relax(1e-10, 1_000_000)
+    st = g.relax_ex(criterion=CRIT_MAX_CHANGE, tolerance=1e-10, max_sweeps=1_000_000,
+                    tile_shape=shape)
+    assert st["sweeps"] == want
+    assert bits_equal(g.download(), ref.values)
+
+
+@pytest.mark.parametrize("shape", [0, 1])
+def test_box_3d_rel_residual(shape):
+    """A 40x36x30 box, faces held, ragged against every tile size."""
+    say(f"case 3d box rel residual shape={shape}")
+    w, h, d = 40, 36, 30
+    rng = np.random.default_rng(3)
+    faces = np.zeros((d, h, w), np.uint8)
+    faces[:, 0, :] = faces[:, -1, :] = faces[:, :, 0] = faces[:, :, -1] = 1
+    faces[0] = faces[-1] = 1
+    ref = OracleField(w, h, d, 0.25, 0.0, 1.0)
+    ref.uptake[:] = np.where(faces.reshape(-1) == 1, 0.0, rng.uniform(0, 2e-3, w * h * d))
+    ref.hold_where(1.0, faces.reshape(-1))
+    ref.copy_next_from_values()
+    g = gpu_from_oracle(ref)
+    st = g.relax_ex(omega=1.85, criterion=CRIT_REL_RESIDUAL, tolerance=1e-10,
+                    max_sweeps=100_000, tile_shape=shape, check_every=1)
+    v = g.download()
+    chk = OracleField(w, h, d, 0.25, 0.0, 1.0)
+    chk.values[:] = v
+    chk.uptake[:] = ref.uptake
+    chk.held[:] = ref.held
+    assert chk.rel_residual()[0] <= 1e-10
+    # the same number of oracle sweeps gives the same bits
+    for _ in range(st["sweeps"]):
+        ref.sor_sweep(1.85)
+    assert bits_equal(v, ref.values)
+
+
+@pytest.mark.parametrize("shape", SHAPES_3D)
+def test_fp32_refine_3d(shape):
+    say(f"case 3d refine shape={shape}")
+    ref, g = ball_case(side=33, r=12.0)
+    ref.relax_residual(1e-13, 1_000_000, omega=1.8)
+    st = g.relax_ex(omega=1.8, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, max_sweeps=200_000,
+                    precision=PREC_FP32_REFINE, tile_shape=shape)
+    v = g.download()
+    chk = OracleField(33, 33, 33, D, 0.0, 1.0)
+    chk.values[:] = v
+    chk.uptake[:] = ref.uptake
+    chk.held[:] = ref.held
+    assert chk.rel_residual()[0] <= 1e-8
+    assert np.max(np.abs(v - ref.values)) < 1e-6
