@@ -31,6 +31,7 @@ class RelaxOpts(C.Structure):
         ("inner_tolerance", C.c_double),
         ("max_outer", C.c_uint64),
         ("tile_shape", C.c_int32),
+        ("use_graph", C.c_int32),
     ]
 
 
@@ -95,6 +96,7 @@ def _declare():
         "cw_field_fill": (st, [P, d]),
         "cw_batch_time_passes": (st, [C.POINTER(P), u32, opts, u32, dp]),
         "cw_field_adjoint": (st, [P, dp, u32p, u32, opts, dp, C.POINTER(AdjointResult)]),
+        "cw_autotune": (st, [C.POINTER(P), u32, opts, u32, C.POINTER(C.c_int32), dp, u32, u32p]),
     })
 
 

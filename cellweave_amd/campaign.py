@@ -1,0 +1,123 @@
+"""Batched campaigns: many independent histories solved together.
+
+The reference runs a campaign's histories in parallel on CPU threads, each with
+its own fields (src/campaign.rs:196-204). Here every field of every history is
+a problem in one batched launch sequence (cw_batch_relax): each launch sweeps
+the tiles of all unconverged fields, and each field stops on its own criterion.
+Fields are grouped by size class, each class with its own tile shape from the
+autotuner table, and steady chunks of passes are replayed from a captured
+hipGraph.
+
+    python -m cellweave_amd.campaign --tune     # on a GPU: writes the shape table
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+
+from .field import (CRIT_REL_RESIDUAL, PREC_FP32_REFINE, PREC_FP64, Field, autotune,
+                    batch_relax)
+
+TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_gfx950.json")
+
+# Size classes by the longest grid edge.
+CLASSES = ((0, 256, "<256"), (256, 512, "256-511"), (512, 1 << 30, ">=512"))
+
+
+def size_class(n: int) -> str:
+    for lo, hi, name in CLASSES:
+        if lo <= n < hi:
+            return name
+    raise ValueError(n)
+
+
+def load_table(path: str = TABLE_PATH) -> dict:
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh)
+    return {}
+
+
+def make_fields(cases_):
+    out = []
+    for fc in cases_:
+        g = Field(fc.w, fc.h, fc.d, fc.diff, fc.decay, fc.initial)
+        g.upload(values=fc.initial_values(), held=fc.held, uptake=fc.uptake,
+                 production=fc.production)
+        out.append(g)
+    return out
+
+
+def omega_for(fc) -> float:
+    import math
+
+    s = fc.span()
+    longest = max(s[1] - s[0] + 1, s[3] - s[2] + 1, s[5] - s[4] + 1)
+    return 2.0 / (1.0 + math.sin(math.pi / max(longest, 2)))
+
+
+def group(cases_):
+    groups = {}
+    for k, fc in enumerate(cases_):
+        groups.setdefault(size_class(max(fc.w, fc.h, fc.d)), []).append(k)
+    return groups
+
+
+def solve(fields, cases_, precision=PREC_FP32_REFINE, tol=1e-8, table=None, use_graph=1,
+          max_sweeps=2_000_000):
+    """Solve every field to `tol`; one batched launch sequence per size class.
+    Returns (stats per field, seconds)."""
+    table = load_table() if table is None else table
+    key = "fp64" if precision == PREC_FP64 else "fp32"
+    stats = [None] * len(fields)
+    t0 = time.perf_counter()
+    for cls, idx in sorted(group(cases_).items()):
+        shape = table.get(key, {}).get(cls, -1)
+        # one relaxation factor per class: the optimum for the class's largest box
+        omega = max(omega_for(cases_[k]) for k in idx)
+        st, _ = batch_relax([fields[k] for k in idx], criterion=CRIT_REL_RESIDUAL, tolerance=tol,
+                            max_sweeps=max_sweeps, omega=omega, precision=precision,
+                            tile_shape=shape, use_graph=use_graph)
+        for k, s in zip(idx, st):
+            stats[k] = s
+    return stats, time.perf_counter() - t0
+
+
+def tune(cases_, per_class=8, launches=10):
+    """Rank the sweep shapes on a sample of each size class, for both precisions."""
+    table = {"fp64": {}, "fp32": {}, "ranking": {}}
+    for cls, idx in sorted(group(cases_).items()):
+        sample = [cases_[k] for k in idx[:per_class]]
+        fields = make_fields(sample)
+        omega = max(omega_for(fc) for fc in sample)
+        for key, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32_REFINE)):
+            ranked = autotune(fields, launches=launches, criterion=CRIT_REL_RESIDUAL,
+                              tolerance=1e-8, omega=omega, precision=prec)
+            table[key][cls] = ranked[0][0]
+            table["ranking"][f"{key} {cls}"] = ranked[:6]
+            for g, fc in zip(fields, sample):
+                g.fill(fc.initial)
+    return table
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--tune", action="store_true")
+    ap.add_argument("--count", type=int, default=64)
+    args = ap.parse_args()
+    from . import cases
+
+    hist = cases.config4(count=args.count)
+    flat = [fc for h in hist for fc in h]
+    if args.tune:
+        table = tune(flat)
+        table["workload"] = f"config 4 sample: {args.count} histories x 2 fields"
+        with open(TABLE_PATH, "w") as fh:
+            json.dump(table, fh, indent=1)
+        print(json.dumps(table))
+
+
+if __name__ == "__main__":
+    main()

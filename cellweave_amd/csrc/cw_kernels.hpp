@@ -97,7 +97,27 @@ struct PassArgs {
     const Ctl* inner_ctl;  // correct64: the inner plan's control blocks
     int project;           // resid64: project onto the floor (0 for the adjoint's linear system)
     int pad_;
+    const int* run;        // graph replays: device base (pass, sweeps) added to pass, sweeps_before
 };
+
+// A launch replayed from a captured graph carries pass and sweeps_before as
+// offsets from a device-resident base that a small kernel advances after each
+// chunk; every kernel resolves them first.
+__device__ __forceinline__ PassArgs resolve(const PassArgs& a0) {
+    PassArgs a = a0;
+    if (a.run) {
+        a.pass += a.run[0];
+        a.sweeps_before += a.run[1];
+    }
+    return a;
+}
+
+__global__ void advance_run(int* run, int passes, int sweeps) {
+    if (threadIdx.x == 0) {
+        run[0] += passes;
+        run[1] += sweeps;
+    }
+}
 
 template <typename T> struct V2;
 template <> struct V2<double> { using type = double2; };
@@ -322,7 +342,8 @@ struct Region {
 // largest change for the reference stopping rule, and optionally the
 // residual of the input for the relative-residual rule.
 template <int S, int RY, int NW, bool GEN>
-__global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a) {
+__global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a_in) {
+    const PassArgs a = resolve(a_in);
     using R = Region<RY, NW, S>;
     constexpr int H = R::H, RX = R::RX, TX = R::TX, TY = R::TY, NR = R::NR;
     static_assert(S <= kMaxS, "change slots");
@@ -490,7 +511,8 @@ __global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a) {
 // the stored coefficient (NaN = held); lb, when present, bounds the FP32
 // correction from below so c + e stays >= 0 (the floor of the field).
 template <typename T, int S, int RY, int NW>
-__global__ __launch_bounds__(NW * 64) void smooth2d(PassArgs a) {
+__global__ __launch_bounds__(NW * 64) void smooth2d(PassArgs a_in) {
+    const PassArgs a = resolve(a_in);
     using R = Region<RY, NW, S>;
     constexpr int H = R::H, RX = R::RX, TX = R::TX, TY = R::TY, NR = R::NR;
     using T2 = typename V2<T>::type;
@@ -625,7 +647,8 @@ __device__ __forceinline__ double from_right(double v) {
 // wave's first and last rows go through LDS, to the waves above and below
 // (double-buffered by half-sweep parity: one barrier per half sweep).
 template <typename T, int S, int NW, int NR>
-__global__ __launch_bounds__(NW * 64) void smooth2d_reg(PassArgs a) {
+__global__ __launch_bounds__(NW * 64) void smooth2d_reg(PassArgs a_in) {
+    const PassArgs a = resolve(a_in);
     constexpr int H = 2 * S;
     constexpr int RX = 128;
     constexpr int RY = NW * NR;
@@ -845,7 +868,8 @@ __device__ __forceinline__ void reg_half(RegCells<T, NR>& R, T above, T below, i
 }
 
 template <typename T, int S, int NW, int NR, bool PLAIN, bool GEN>
-__global__ __launch_bounds__(NW * 64) void relax2d_reg(PassArgs a) {
+__global__ __launch_bounds__(NW * 64) void relax2d_reg(PassArgs a_in) {
+    const PassArgs a = resolve(a_in);
     constexpr int H = 2 * S;
     constexpr int RX = 128;
     constexpr int RY = NW * NR;
@@ -1122,7 +1146,8 @@ __device__ __forceinline__ void plane_update(Plane<T, NR>& cur, const Plane<T, N
 }
 
 template <typename T, int NW, int NR, bool PLAIN, bool GEN>
-__global__ __launch_bounds__(NW * 64) void relax3d_stream(PassArgs a) {
+__global__ __launch_bounds__(NW * 64) void relax3d_stream(PassArgs a_in) {
+    const PassArgs a = resolve(a_in);
     constexpr int RY = NW * NR;
     constexpr int TX = 124, TY = RY - 4;
     static_assert(NR >= 2 && TY > 0, "tile shape");
@@ -1302,7 +1327,8 @@ __device__ __forceinline__ ElTile el_tile(const Prob& pr, int t) {
 // the reference's read-old/write-new half sweep (pde.rs:322-328). Red and
 // black launches of one pass carry the same pass index.
 template <typename T, bool PLAIN>
-__global__ __launch_bounds__(256) void half3d(PassArgs a, int colour) {
+__global__ __launch_bounds__(256) void half3d(PassArgs a_in, int colour) {
+    const PassArgs a = resolve(a_in);
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int p = a.tile_prob[t];
     const Prob& pr = a.probs[p];
@@ -1458,7 +1484,8 @@ __global__ __launch_bounds__(256) void correct64(PassArgs a) {
 // launch of a chunk (as the next sweep launch would); DECIDE_RESID turns the
 // resid64 partials into r2, b2 and stops converged problems; DECIDE_REPLAY_END
 // closes replayed problems.
-__global__ __launch_bounds__(256) void decide(PassArgs a) {
+__global__ __launch_bounds__(256) void decide(PassArgs a_in) {
+    const PassArgs a = resolve(a_in);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     for (int p = wv; p < a.nprob; p += nw) {
         const Prob& pr = a.probs[p];

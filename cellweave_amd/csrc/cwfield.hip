@@ -248,8 +248,17 @@ struct Plan {
     size_t cap_prob = 0, cap_ctl = 0, cap_tp[3] = {0, 0, 0}, cap_part = 0;
     // host mirror of the pass sequence of the current run
     int pass = 0, sweeps_before = 0, prev_s = 0, prev_tl = 0, prev_check = 0;
+    // captured chunk of passes (hipGraph), replayed with a device-resident base
+    bool use_graph = false;
+    int* run = nullptr;
+    int run_host[2] = {0, 0};
+    hipGraphExec_t gexec = nullptr;
+    int g_k = 0, g_crit = -1, g_ce = 0, g_max = 0;
+    double g_tol = 0.0;
 
     ~Plan() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        (void)hipFree(run);
         (void)hipFree(dp); (void)hipFree(dc); (void)hipFree(partials);
         for (int i = 0; i < 3; i++) (void)hipFree(dtp[i]);
     }
@@ -505,7 +514,69 @@ cw_status one_pass(Plan& pl, int s, int crit, int check, double tol, int max_swe
     return CW_OK;
 }
 
-struct LoopResult { long passes = 0; };
+struct LoopResult { long passes = 0; long graph_replays = 0; };
+
+// Captures K passes (launch j carrying pass offset j), the end-of-chunk
+// verdict and the advance of the device base into one hipGraph; cached per
+// plan for the same pattern.
+cw_status capture_chunk(Plan& pl, int K, int crit, int ce, double tol, int maxs) {
+    const int S = pl.three_d ? 1 : pl.s_main;
+    if (pl.gexec && pl.g_k == K && pl.g_crit == crit && pl.g_ce == ce && pl.g_tol == tol &&
+        pl.g_max == maxs)
+        return CW_OK;
+    if (pl.gexec) {
+        (void)hipGraphExecDestroy(pl.gexec);
+        pl.gexec = nullptr;
+    }
+    if (!pl.run) CW_HIP(hipMalloc((void**)&pl.run, 2 * sizeof(int)));
+    CW_HIP(hipStreamBeginCapture(pl.stream, hipStreamCaptureModeThreadLocal));
+    cw_status st = CW_OK;
+    auto pattern_check = [&](int j) { return crit == CRIT_RESID && (j % ce == 0) ? 1 : 0; };
+    for (int j = 0; j < K && st == CW_OK; j++) {
+        const int tl = pl.three_d && pl.in_place() ? TL_EL : TL_MAIN;
+        PassArgs a = make_args(pl, tl, MODE_SWEEP, crit, tol, maxs);
+        a.pass = j;
+        a.sweeps_before = j * S;
+        a.s = S;
+        a.prev_s = S;
+        a.prev_tl = TL_MAIN;
+        a.prev_check = pl.three_d ? 0 : pattern_check((j + K - 1) % K);
+        a.check = pl.three_d ? 0 : pattern_check(j);
+        a.run = pl.run;
+        if (pl.three_d && pl.in_place()) {
+            st = launch_half3d(pl, a, 0);
+            if (st == CW_OK) st = launch_half3d(pl, a, 1);
+        } else if (pl.three_d) {
+            st = launch_stream3d(pl, a);
+        } else {
+            st = launch_sweep(pl, a);
+        }
+    }
+    if (st == CW_OK) {
+        PassArgs a = make_args(pl, TL_MAIN, DECIDE_PASS, crit, tol, maxs);
+        a.pass = K;
+        a.sweeps_before = K * S;
+        a.prev_s = S;
+        a.prev_tl = TL_MAIN;
+        a.prev_check = pl.three_d ? 0 : pattern_check(K - 1);
+        a.run = pl.run;
+        hipLaunchKernelGGL(decide, dim3(1), dim3(256), 0, pl.stream, a);
+        hipLaunchKernelGGL(advance_run, dim3(1), dim3(64), 0, pl.stream, pl.run, K, K * S);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(pl.stream, &g);
+    if (st) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (ec != hipSuccess) return fail(CW_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&pl.gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess)
+        return fail(CW_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(ei));
+    pl.g_k = K; pl.g_crit = crit; pl.g_ce = ce; pl.g_tol = tol; pl.g_max = maxs;
+    return CW_OK;
+}
 
 // Runs passes until no problem is RUNNING or `budget` sweeps have run; the
 // device stops a problem once the run has done `dev_max` sweeps. Problems
@@ -518,7 +589,35 @@ cw_status run_passes(Plan& pl, int crit, double tol, uint64_t budget, uint64_t d
     const uint64_t start = (uint64_t)pl.sweeps_before;
     long launched = 0;
     int chunk = 4;
+    const int S = pl.three_d ? 1 : pl.s_main;
+    constexpr int kGraphPasses = 16;
     while (true) {
+        // steady state: replay a captured chunk of passes
+        const uint64_t done0 = (uint64_t)pl.sweeps_before - start;
+        const bool steady =
+            pl.pass == 0 ||
+            (pl.prev_s == S && pl.prev_tl == TL_MAIN &&
+             pl.prev_check == ((crit == CRIT_RESID && !pl.three_d && ((pl.pass - 1) % check_every == 0)) ? 1 : 0));
+        if (pl.use_graph && steady && kGraphPasses % check_every == 0 &&
+            pl.pass % check_every == 0 && done0 + (uint64_t)(kGraphPasses * S) <= budget &&
+            !(pl.three_d && crit == CRIT_RESID)) {
+            if ((st = capture_chunk(pl, kGraphPasses, crit, check_every, tol, maxs))) return st;
+            pl.run_host[0] = pl.pass;
+            pl.run_host[1] = pl.sweeps_before;
+            CW_HIP(hipMemcpyAsync(pl.run, pl.run_host, 2 * sizeof(int), hipMemcpyHostToDevice,
+                                  pl.stream));
+            CW_HIP(hipGraphLaunch(pl.gexec, pl.stream));
+            pl.pass += kGraphPasses;
+            pl.sweeps_before += kGraphPasses * S;
+            pl.prev_s = S;
+            pl.prev_tl = TL_MAIN;
+            pl.prev_check = (crit == CRIT_RESID && !pl.three_d && ((kGraphPasses - 1) % check_every == 0)) ? 1 : 0;
+            launched += kGraphPasses;
+            if (out) out->graph_replays++;
+            if ((st = read_ctl(pl))) return st;
+            if (!any_status(pl, ST_RUNNING)) break;
+            continue;
+        }
         for (int j = 0; j < chunk; j++) {
             const uint64_t done = (uint64_t)pl.sweeps_before - start;
             if (done >= budget) break;
@@ -588,7 +687,8 @@ struct Opts {
     int check_every;
     double inner_tol;
     uint64_t max_outer;
-    int shape = -1;            // kShapes index forced for the smoother (-1: from s)
+    int shape = -1;            // tile shape index forced by the caller (-1: from s)
+    bool use_graph = false;    // replay steady chunks of passes from a captured hipGraph
     long time_passes = 0;      // > 0: time this many sweep launches instead of solving
     double* time_ms = nullptr; // average duration of one sweep launch (HIP events)
 };
@@ -651,6 +751,7 @@ cw_status check_opts(const cw_relax_opts* o, Opts* out) {
     out->inner_tol = o->inner_tolerance > 0.0 ? o->inner_tolerance : 1e-3;
     out->max_outer = o->max_outer > 0 ? o->max_outer : 1000;
     out->shape = o->tile_shape;
+    out->use_graph = o->use_graph != 0;
     return CW_OK;
 }
 
@@ -677,6 +778,7 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     pl.shape = o.shape >= 0 ? o.shape : default_plain_shape(s);
     if (pl.shape >= kNumPlainShapes) return fail(CW_ERR_ARG, "unknown tile shape %d", pl.shape);
     pl.s_main = kPlainShapes[pl.shape].s;
+    pl.use_graph = o.use_graph;
     if (three_d) {
         // volumes: tile_shape picks a 3D shape, one sweep per launch
         pl.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape : kDefault3DShape;
@@ -822,6 +924,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     outer.s_main = 1;
     inner.s_main = s;
     inner.shape = shape;
+    inner.use_graph = o.use_graph;
     if (three_d) inner.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape : kDefault3DShape;
     std::vector<int> idx;
     std::vector<double> omegas(nf);
@@ -1209,6 +1312,7 @@ void cw_relax_opts_default(cw_relax_opts* o) {
     o->inner_tolerance = 0.0;
     o->max_outer = 0;
     o->tile_shape = -1;
+    o->use_graph = 0;
 }
 
 const char* cw_last_error(void) { return g_err.c_str(); }
@@ -1342,6 +1446,36 @@ cw_status cw_batch_time_passes(cw_field* const* fields, uint32_t count, const cw
     if (!fields || count == 0 || !opts || !ms_per_pass || passes == 0)
         return fail(CW_ERR_ARG, "bad argument");
     return solve(fields, count, opts, nullptr, (long)passes, ms_per_pass);
+}
+
+cw_status cw_autotune(cw_field* const* fields, uint32_t count, const cw_relax_opts* opts,
+                      uint32_t launches, int32_t* shapes, double* us_per_sweep, uint32_t cap,
+                      uint32_t* n_out) {
+    if (!fields || count == 0 || !opts || !shapes || !us_per_sweep || !n_out || launches == 0)
+        return fail(CW_ERR_ARG, "bad argument");
+    const bool three_d = fields[0]->D > 1;
+    const bool plain = opts->precision == CW_PREC_FP64;
+    const int n = three_d ? kNum3DShapes : plain ? kNumPlainShapes : kNumSmoothShapes;
+    std::vector<std::pair<double, int>> res;
+    for (int sh = 0; sh < n; sh++) {
+        cw_relax_opts o = *opts;
+        o.tile_shape = sh;
+        o.sweeps_per_pass = 0;
+        o.use_graph = 0;
+        double ms = 0.0;
+        cw_status st = solve(fields, count, &o, nullptr, (long)launches, &ms);
+        if (st) return st;
+        const int S = three_d ? 1 : plain ? kPlainShapes[sh].s : kSmoothShapes[sh].s;
+        res.push_back({1e3 * ms / S, sh});
+    }
+    std::sort(res.begin(), res.end());
+    uint32_t k = 0;
+    for (; k < res.size() && k < cap; k++) {
+        shapes[k] = res[k].second;
+        us_per_sweep[k] = res[k].first;
+    }
+    *n_out = k;
+    return CW_OK;
 }
 
 cw_status cw_field_fill(cw_field* f, double value) {

@@ -55,7 +55,7 @@ def _dptr(a):
 
 def make_opts(omega=0.0, criterion=CRIT_MAX_CHANGE, tolerance=1e-8, max_sweeps=2_000_000,
               precision=PREC_FP64, sweeps_per_pass=0, check_every=0, inner_tolerance=0.0,
-              max_outer=0, tile_shape=-1) -> RelaxOpts:
+              max_outer=0, tile_shape=-1, use_graph=0) -> RelaxOpts:
     o = RelaxOpts()
     _abi.lib().cw_relax_opts_default(C.byref(o))
     o.omega = omega
@@ -68,6 +68,7 @@ def make_opts(omega=0.0, criterion=CRIT_MAX_CHANGE, tolerance=1e-8, max_sweeps=2
     o.inner_tolerance = inner_tolerance
     o.max_outer = max_outer
     o.tile_shape = tile_shape
+    o.use_graph = use_graph
     return o
 
 
@@ -183,6 +184,19 @@ def time_passes(fields, passes, **kw) -> float:
     return ms.value
 
 
-__all__ = ["time_passes", "Field", "batch_relax", "make_opts", "CellweaveError", "ConfigError",
+def autotune(fields, launches=10, **kw):
+    """Rank every sweep-kernel shape for these fields: [(shape, us per sweep), ...]."""
+    o = make_opts(**kw)
+    n = len(fields)
+    arr = (C.c_void_p * n)(*[f._h.value for f in fields])
+    cap = 64
+    shapes = (C.c_int32 * cap)()
+    us = (C.c_double * cap)()
+    got = C.c_uint32()
+    _check(_abi.lib().cw_autotune(arr, n, C.byref(o), launches, shapes, us, cap, C.byref(got)))
+    return [(int(shapes[i]), float(us[i])) for i in range(got.value)]
+
+
+__all__ = ["autotune", "time_passes", "Field", "batch_relax", "make_opts", "CellweaveError", "ConfigError",
            "SimulationError", "DeviceError", "CRIT_MAX_CHANGE", "CRIT_REL_RESIDUAL",
            "PREC_FP64", "PREC_FP32_REFINE"]

@@ -67,8 +67,10 @@ typedef struct cw_relax_opts {
     int32_t check_every;      /* residual test every k passes; 0 = library choice */
     double inner_tolerance;   /* FP32_REFINE: inner reduction per outer step; 0 = default */
     uint64_t max_outer;       /* FP32_REFINE: outer iteration cap; 0 = default */
-    int32_t tile_shape;       /* FP32 smoother tile shape index (-1 = from
-                                 sweeps_per_pass); cw_autotune picks it */
+    int32_t tile_shape;       /* sweep kernel shape index (-1 = from
+                                 sweeps_per_pass); cw_autotune ranks them */
+    int32_t use_graph;        /* 1: replay steady chunks of passes from a
+                                 captured hipGraph (one launch per chunk) */
 } cw_relax_opts;
 
 typedef struct cw_relax_stats {
@@ -164,6 +166,14 @@ typedef struct cw_adjoint_result {
 cw_status cw_field_adjoint(cw_field *field, const double *target, const uint32_t *grid,
                            uint32_t ncells, const cw_relax_opts *opts, double *dl_du,
                            cw_adjoint_result *result);
+
+/* Autotuner (no reference counterpart): times `launches` back-to-back launches
+ * of every sweep-kernel shape that fits these fields and this precision
+ * (2D or 3D) and writes up to `cap` shapes ranked by microseconds per
+ * sweep. The fields' values are swept by the call. */
+cw_status cw_autotune(cw_field *const *fields, uint32_t count, const cw_relax_opts *opts,
+                      uint32_t launches, int32_t *shapes, double *us_per_sweep, uint32_t cap,
+                      uint32_t *n_out);
 
 /* Measurement helper (no reference counterpart): builds the same launch plan
  * cw_batch_relax would for `opts` and times `passes` back-to-back launches of

@@ -1,0 +1,69 @@
+"""Batched and graph-replayed solves: the same bits as field-by-field eager
+solves, every replay checked against the eager result."""
+import numpy as np
+import pytest
+
+from cellweave_amd import campaign, cases
+from cellweave_amd.field import (CRIT_MAX_CHANGE, CRIT_REL_RESIDUAL, PREC_FP32_REFINE,
+                                 PREC_FP64, batch_relax)
+from helpers import bits_equal, gpu_from_case
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("crit,tol", [(CRIT_MAX_CHANGE, 1e-10), (CRIT_REL_RESIDUAL, 1e-9)])
+def test_graph_replay_equals_eager(crit, tol):
+    say(f"case graph vs eager crit={crit}")
+    fcs = cases.config2(n=192) + cases.config1()
+    eager = [gpu_from_case(fc) for fc in fcs]
+    st_e, _ = batch_relax(eager, criterion=crit, tolerance=tol, max_sweeps=200_000, omega=1.95)
+    graph = [gpu_from_case(fc) for fc in fcs]
+    # two solves in a row from the same start: the cached graph is replayed again
+    for rep in range(2):
+        for g, fc in zip(graph, fcs):
+            g.upload(values=fc.initial_values())
+        st_g, _ = batch_relax(graph, criterion=crit, tolerance=tol, max_sweeps=200_000,
+                              omega=1.95, use_graph=1)
+        for a, b, ge, gg in zip(st_e, st_g, eager, graph):
+            assert a["sweeps"] == b["sweeps"], rep
+            assert bits_equal(ge.download(), gg.download()), rep
+
+
+def test_graph_refine_equals_eager():
+    say("case graph refine")
+    fcs = cases.config2(n=192)
+    e = [gpu_from_case(fc) for fc in fcs]
+    g = [gpu_from_case(fc) for fc in fcs]
+    st_e, _ = batch_relax(e, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, omega=1.96,
+                          precision=PREC_FP32_REFINE)
+    st_g, _ = batch_relax(g, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, omega=1.96,
+                          precision=PREC_FP32_REFINE, use_graph=1)
+    for a, b, x, y in zip(st_e, st_g, e, g):
+        assert a["sweeps"] == b["sweeps"]
+        assert bits_equal(x.download(), y.download())
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_FP32_REFINE])
+def test_small_campaign_converges_and_matches_single_solves(precision):
+    say(f"case small campaign precision={precision}")
+    hist = cases.config4(count=12, lo=48, hi=160)
+    flat = [fc for h in hist for fc in h]
+    fields = campaign.make_fields(flat)
+    stats, secs = campaign.solve(fields, flat, precision=precision, tol=1e-8, table={})
+    for fc, st, f in zip(flat, stats, fields):
+        assert st["status"] == 0
+        assert st["rel_residual"] <= 1e-8
+    if precision == PREC_FP64:
+        # the batch equals each field solved alone with the class's factor
+        groups = campaign.group(flat)
+        for cls, idx in groups.items():
+            omega = max(campaign.omega_for(flat[k]) for k in idx)
+            for k in idx[:3]:
+                g = gpu_from_case(flat[k])
+                st = g.relax_ex(criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, omega=omega)
+                assert st["sweeps"] == stats[k]["sweeps"]
+                assert bits_equal(g.download(), fields[k].download())
