@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--time-launches", type=int, default=200)
+    ap.add_argument("--workload", default="config2",
+                    help="config2 (default line) or config1/config3/config4/config5")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
@@ -95,6 +97,12 @@ def main():
         sys.exit(2)
 
     import numpy as np
+
+    if args.workload != "config2":
+        from cellweave_amd import workloads
+
+        print(json.dumps(workloads.WORKLOADS[args.workload]()))
+        return
 
     from cellweave_amd import cases
     from cellweave_amd.field import (CRIT_REL_RESIDUAL, PREC_FP32_REFINE, PREC_FP64, Field,

@@ -1,0 +1,313 @@
+"""The measured workloads beside bench.py's default line.
+
+Each function runs one workload on the GPU and returns a dict in the shape of
+bench.py's JSON line: voxel-sweeps/s (swept voxels x sweeps / wall time), the
+FP64 solve beside the FP32+refine one, and the live per-launch timing of the
+dominant kernel for the roofline.
+
+    config1: 200^2, one oxygen field, omega 1.9 (the reference's CPU-sized case)
+    config3: 256^3 spheroid, FP32+refine against FP64
+    config4: 512 histories x 2 fields, batched per size class, graph replays
+    config5: 192^3 adjoint, gradients against central finite differences
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+
+from . import campaign, cases
+from .field import (CRIT_REL_RESIDUAL, PREC_FP32_REFINE, PREC_FP64, Field,
+                    batch_relax, time_passes)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _field(fc):
+    g = Field(fc.w, fc.h, fc.d, fc.diff, fc.decay, fc.initial)
+    g.upload(values=fc.initial_values(), held=fc.held, uptake=fc.uptake,
+             production=fc.production)
+    return g
+
+
+def _line(metric_value, config, extra):
+    out = {
+        "metric": "voxel-sweeps/s & % HBM roofline, FP32+refine vs FP64 SOR at 1e-8 residual",
+        "value": metric_value, "unit": "voxel-sweeps/s", "n_gpus": 1,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "data": "synthetic (seeded Voronoi lattice)", "config": config,
+    }
+    out.update(extra)
+    return out
+
+
+def _solve_pair(fields, fcs, omega, reset, tol=1e-8, reps=3, use_graph=0):
+    span = [fc.span_voxels() for fc in fcs]
+    res = {}
+    for name, prec in (("fp64", PREC_FP64), ("fp32_refine", PREC_FP32_REFINE)):
+        reset()
+        batch_relax(fields, criterion=CRIT_REL_RESIDUAL, tolerance=tol, omega=omega,
+                    precision=prec, use_graph=use_graph)  # warm-up
+        ts, st = [], None
+        for _ in range(reps):
+            reset()
+            t0 = time.perf_counter()
+            st, _ = batch_relax(fields, criterion=CRIT_REL_RESIDUAL, tolerance=tol, omega=omega,
+                                precision=prec, use_graph=use_graph)
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        vs = sum(s * x["sweeps"] for s, x in zip(span, st))
+        res[name] = {"ms_per_solve": 1e3 * t, "voxel_sweeps_per_s": vs / t,
+                     "sweeps": [x["sweeps"] for x in st],
+                     "rel_residual": [x["rel_residual"] for x in st],
+                     "outer": [x["outer_iters"] for x in st],
+                     "values": [g.download() for g in fields]}
+    return res
+
+
+def _errors(res, fcs):
+    out = {}
+    for fc, a, b in zip(fcs, res["fp32_refine"]["values"], res["fp64"]["values"]):
+        out[fc.name] = {"max_abs": float(np.max(np.abs(a - b))),
+                        "rel_l2": float(np.linalg.norm(a - b) / (np.linalg.norm(b) or 1.0))}
+    return out
+
+
+def _strip(res):
+    return {k: {kk: vv for kk, vv in v.items() if kk != "values"} for k, v in res.items()}
+
+
+def config1():
+    (fc,) = cases.config1()
+    g = _field(fc)
+    res = _solve_pair([g], [fc], 1.9, lambda: g.fill(1.0))
+    vs = res["fp32_refine"]["voxel_sweeps_per_s"]
+    return _line(vs, {"workload": "200x200 oxygen field, ~450 cells, omega 1.9, rel residual 1e-8"},
+                 {"fp64": _strip(res)["fp64"], "fp32_refine": _strip(res)["fp32_refine"],
+                  "errors_vs_fp64": _errors(res, [fc])})
+
+
+def config3(n=256):
+    t0 = time.perf_counter()
+    (fc,) = cases.config3(n=n, ncells=int(round(60000 * (n / 256) ** 3)), radius=110 * n / 256,
+                          necrotic=40 * n / 256)
+    build = time.perf_counter() - t0
+    g = _field(fc)
+    omega = campaign.omega_for(fc)
+    res = _solve_pair([g], [fc], omega, lambda: g.fill(1.0), reps=2)
+    span = fc.span_voxels()
+    g.fill(1.0)
+    ms32 = time_passes([g], 40, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, omega=omega,
+                       precision=PREC_FP32_REFINE)
+    g.fill(1.0)
+    ms64 = time_passes([g], 40, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, omega=omega,
+                       precision=PREC_FP64)
+    b32 = span * 16.0  # e 4+4, rhs 4, lower bound 4 (oxygen has sinks), one sweep per launch
+    b64 = span * 24.0
+    s = _strip(res)
+    return _line(res["fp32_refine"]["voxel_sweeps_per_s"],
+                 {"workload": f"{n}^3 spheroid, ~60k cells, FP32+refine vs FP64 to 1e-8",
+                  "omega": omega, "swept_voxels": span, "case_build_s": build},
+                 {"fp64": s["fp64"], "fp32_refine": s["fp32_refine"],
+                  "errors_vs_fp64": _errors(res, [fc]),
+                  "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                               "fp32_kernel_launch_ms": ms32,
+                               "fp32_achieved": b32 / ms32 / 1e6,
+                               "fp32_frac": b32 / ms32 / 1e6 / HBM_PEAK_GBS,
+                               "fp64_kernel_launch_ms": ms64,
+                               "fp64_achieved": b64 / ms64 / 1e6,
+                               "fp64_frac": b64 / ms64 / 1e6 / HBM_PEAK_GBS},
+                  "speedup_vs_fp64": {
+                      "throughput": res["fp32_refine"]["voxel_sweeps_per_s"] /
+                      res["fp64"]["voxel_sweeps_per_s"],
+                      "time_to_solution": res["fp64"]["ms_per_solve"] /
+                      res["fp32_refine"]["ms_per_solve"]}})
+
+
+def config4(count=512):
+    t0 = time.perf_counter()
+    hist = cases.config4(count=count)
+    flat = [fc for h in hist for fc in h]
+    build = time.perf_counter() - t0
+    fields = campaign.make_fields(flat)
+    span = [fc.span_voxels() for fc in flat]
+    table = campaign.load_table()
+    out = {}
+    for name, prec in (("fp64", PREC_FP64), ("fp32_refine", PREC_FP32_REFINE)):
+        for g, fc in zip(fields, flat):
+            g.fill(fc.initial)
+        stats, secs = campaign.solve(fields, flat, precision=prec, table=table)
+        vs = sum(s * st["sweeps"] for s, st in zip(span, stats))
+        worst = max(st["rel_residual"] for st in stats)
+        out[name] = {"seconds": secs, "voxel_sweeps_per_s": vs / secs,
+                     "total_sweeps": int(sum(st["sweeps"] for st in stats)),
+                     "worst_rel_residual": worst,
+                     "all_converged": all(st["status"] == 0 for st in stats)}
+    return _line(out["fp32_refine"]["voxel_sweeps_per_s"],
+                 {"workload": f"{count} histories x 2 fields (oxygen, signal), edges "
+                              "logU[128,768], batched per size class with graph replays",
+                  "fields": len(flat), "voxels": int(sum(fc.n for fc in flat)),
+                  "case_build_s": build, "shape_table": {k: table.get(k) for k in ("fp64", "fp32")}},
+                 {"fp64": out["fp64"], "fp32_refine": out["fp32_refine"],
+                  "speedup_vs_fp64": {"throughput": out["fp32_refine"]["voxel_sweeps_per_s"] /
+                                      out["fp64"]["voxel_sweeps_per_s"],
+                                      "time_to_solution": out["fp64"]["seconds"] /
+                                      out["fp32_refine"]["seconds"]}})
+
+
+def neighbour_defect(c, w, h, d):
+    """q_i = sum over in-box neighbours j of (c_i - c_j): the operator of the
+    steady state D q(c) + decay c = source + p - u (pde.rs:384-395) without D."""
+    c3 = c.reshape(d, h, w)
+    q = np.zeros_like(c3)
+    for ax in range(3):
+        if c3.shape[ax] < 2:
+            continue
+        hi = [slice(None)] * 3
+        lo = [slice(None)] * 3
+        hi[ax] = slice(1, None)
+        lo[ax] = slice(None, -1)
+        step = c3[tuple(hi)] - c3[tuple(lo)]
+        q[tuple(hi)] += step
+        q[tuple(lo)] -= step
+    return q.reshape(-1)
+
+
+class SpheroidAdjoint:
+    """The adjoint workload on an n^3 Voronoi spheroid (oxygen, D = 20 px^2/s,
+    no decay, box faces held at g = 1, necrotic core without uptake):
+    loss 0.5 sum (c - target)^2 with the target the FP64 steady state at
+    D x 1.1 and uptakes x (1 + 0.1 U[-1, 1]).
+
+    Central differences at relative step h solve each perturbed problem in
+    correction form, c = base + delta, for delta' = delta + shift with
+    D q(delta') = -u - D q(base) and held delta' = shift + dg. Solved directly,
+    c ~ 1 carries 1e-16 rounding while a 1e-6 change of one cell's uptake moves
+    c by ~1e-10, which limits the quotient to ~1e-5; in correction form the
+    rounding scales with the shift (4h) instead. The shift keeps delta' above
+    the solve's floor at zero, so the projected solve returns the linear one:
+    with the floor inactive c = 1 - L^-1 u / D (L the graph Laplacian with
+    the faces held), so a step h in D or in one cell's rate moves c by at most
+    h max(1 - c), and a step h in g moves it by exactly h."""
+
+    def __init__(self, n=192, ncells=25000, radius=82.0, necrotic=40.0):
+        self.n = n
+        self.fc, self.pert = cases.config5(n=n, ncells=ncells, radius=radius, necrotic=necrotic)
+        self.necro = cases.necrotic_mask(n, necrotic)
+        self.held = self.fc.held == 1
+        self.rates = self.fc.per_cell_uptake
+        self.omega = campaign.omega_for(self.fc)
+        assert np.array_equal(self.uptake_of(self.rates), np.where(self.held, 0.0, self.fc.uptake))
+
+    def uptake_of(self, rates):
+        up = cases.gather(self.fc.grid, rates)
+        up[self.necro | self.held] = 0.0
+        return up
+
+    def solve(self, diff, g, rates, precision=PREC_FP64, tol=1e-13):
+        n = self.n
+        f = Field(n, n, n, diff, 0.0, 1.0)
+        f.upload(values=np.where(self.held, g, 1.0), held=self.fc.held,
+                 uptake=self.uptake_of(rates))
+        st = f.relax_ex(omega=self.omega, criterion=CRIT_REL_RESIDUAL, tolerance=tol,
+                        precision=precision, max_sweeps=400_000)
+        return f, st
+
+    def prepare(self):
+        fc = self.fc
+        ft, _ = self.solve(fc.diff * self.pert["diff_scale"], 1.0,
+                           self.rates * self.pert["uptake_scale"])
+        self.target = ft.download()
+        fb, self.base_stats = self.solve(fc.diff, 1.0, self.rates)
+        self.base = fb.download()
+        self.q = neighbour_defect(self.base, self.n, self.n, self.n)
+
+    def delta(self, diff, dg, rates, shift, tol=1e-13):
+        n = self.n
+        r = np.where(self.held, 0.0, -self.uptake_of(rates) - diff * self.q)
+        f = Field(n, n, n, diff, 0.0, shift)
+        f.upload(values=np.where(self.held, shift + dg, shift), held=self.fc.held,
+                 uptake=np.where(r < 0, -r, 0.0), production=np.where(r > 0, r, 0.0))
+        st = f.relax_ex(omega=self.omega, criterion=CRIT_REL_RESIDUAL, tolerance=tol,
+                        precision=PREC_FP64, max_sweeps=400_000)
+        out = f.download()
+        if out.min() <= 0.0:
+            raise ValueError("the shifted correction touched the floor; raise the shift")
+        return out - shift, st["sweeps"]
+
+    def quotient(self, plus, minus, den):
+        (dp, sp), (dm, sm) = plus, minus
+        self.fd_sweeps += [sp, sm]
+        return 0.5 * math.fsum((dp - dm) * (2.0 * (self.base - self.target) + dp + dm)) / den
+
+    def differences(self, h=1e-6, cells=()):
+        """Central differences of the loss in D, g and each listed cell's rate."""
+        self.fd_sweeps = []
+        D, r0 = self.fc.diff, self.rates
+        s = 4.0 * h * float(np.max(1.0 - self.base))  # bounds |delta| for D and cells
+        sg = 4.0 * h
+        out = {"D": self.quotient(self.delta(D * (1 + h), 0.0, r0, s),
+                                  self.delta(D * (1 - h), 0.0, r0, s), 2 * h * D),
+               "g": self.quotient(self.delta(D, h, r0, sg), self.delta(D, -h, r0, sg), 2 * h),
+               "cells": {}}
+        for k in cells:
+            up, dn = r0.copy(), r0.copy()
+            up[k] *= 1 + h
+            dn[k] *= 1 - h
+            out["cells"][int(k)] = self.quotient(self.delta(D, 0.0, up, s),
+                                                 self.delta(D, 0.0, dn, s), 2 * h * r0[k])
+        return out
+
+
+def config5(n=192, ncells=25000, radius=82.0, fd_cells=8, h=1e-6, adj_tol=1e-11):
+    """Forward in FP32+refinement, adjoint in FP64, gradients against central
+    differences at relative step h (correction-form perturbed solves)."""
+    t0 = time.perf_counter()
+    case = SpheroidAdjoint(n=n, ncells=ncells, radius=radius, necrotic=40.0 * n / 192)
+    build = time.perf_counter() - t0
+    case.prepare()
+    rates = case.rates
+    # the measured path: forward FP32+refinement, adjoint FP64
+    t1 = time.perf_counter()
+    f, st_fwd = case.solve(case.fc.diff, 1.0, rates, precision=PREC_FP32_REFINE, tol=1e-12)
+    t_fwd = time.perf_counter() - t1
+    c = f.download()
+    # The adjoint's right-hand side c - target is interior and small, so its
+    # residual floor sits near eps * cond(A) (3e-13..1e-12 at 64^3, ~9x that
+    # at 192^3); the gradients agree to 1e-13 between 1e-11 and 1e-12 stops.
+    t1 = time.perf_counter()
+    adj = f.adjoint(case.target, grid=case.fc.grid, ncells=rates.size, tolerance=adj_tol,
+                    omega=case.omega, max_sweeps=200_000)
+    t_adj = time.perf_counter() - t1
+    del f
+    live = np.flatnonzero(np.abs(adj["du"]) > 0)
+    cells = np.random.default_rng(5005).choice(live, size=min(fd_cells, live.size), replace=False)
+    t1 = time.perf_counter()
+    fd = case.differences(h=h, cells=cells)
+    t_fd = time.perf_counter() - t1
+    rel = lambda a, b: abs(a - b) / abs(b)
+    checks = {"D": {"adjoint": adj["dl_dd"], "fd": fd["D"], "rel_err": rel(adj["dl_dd"], fd["D"])},
+              "g": {"adjoint": adj["dl_dg"], "fd": fd["g"], "rel_err": rel(adj["dl_dg"], fd["g"])}}
+    cell_err = {k: rel(adj["du"][k], v) for k, v in fd["cells"].items()}
+    checks["uptake"] = {"cells": {str(k): {"adjoint": float(adj["du"][k]), "fd": v,
+                                           "rel_err": cell_err[k]} for k, v in fd["cells"].items()},
+                        "max_rel_err": max(cell_err.values())}
+    worst = max(checks["D"]["rel_err"], checks["g"]["rel_err"], checks["uptake"]["max_rel_err"])
+    return {"workload": f"{n}^3 oxygen volume, {rates.size - 1} cells, loss 0.5 sum (c - c_target)^2, "
+                        "target from D x 1.1 and uptake x (1 + 0.1 U[-1,1])",
+            "omega": case.omega, "case_build_s": build, "min_c": float(c.min()),
+            "forward": {"precision": "fp32_refine", "seconds": t_fwd, "sweeps": st_fwd["sweeps"],
+                        "outer": st_fwd["outer_iters"], "rel_residual": st_fwd["rel_residual"],
+                        "max_abs_vs_fp64": float(np.max(np.abs(c - case.base))),
+                        "fp64_sweeps": case.base_stats["sweeps"]},
+            "adjoint": {"precision": "fp64", "tolerance": adj_tol, "seconds": t_adj,
+                        "sweeps": adj["sweeps"], "rel_residual": adj["rel_residual"],
+                        "loss": adj["loss"]},
+            "fd": {"h_rel": h, "form": "correction (c = base + delta, shift 4h max(1-c), 4h for g)", "solve_tol": 1e-13,
+                   "seconds": t_fd, "sweeps": case.fd_sweeps},
+            "checks": checks, "worst_rel_err": worst, "within_1e-6": bool(worst <= 1e-6),
+            "units": "dL/dD per px^2/s (the solve's D is um^2/s divided by dx^2 = 100 um^2)"}
+
+
+WORKLOADS = {"config1": config1, "config3": config3, "config4": config4, "config5": config5}

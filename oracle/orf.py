@@ -120,22 +120,29 @@ class OracleField:
             lib().orf_destroy(h)
             self._h = None
 
-    # views onto the C arrays (valid until the next sweep swaps them)
+    # Views onto the C arrays, valid until the next sweep swaps values and
+    # next. Each view holds a reference to this field, so that an expression
+    # such as `make_field().values.copy()` cannot free the memory under it.
+    def _view(self, ptr, ctype) -> np.ndarray:
+        buf = (ctype * self.n).from_address(C.cast(ptr, C.c_void_p).value)
+        buf._owner = self
+        return np.ctypeslib.as_array(buf)
+
     @property
     def values(self) -> np.ndarray:
-        return np.ctypeslib.as_array(lib().orf_values(self._h), shape=(self.n,))
+        return self._view(lib().orf_values(self._h), C.c_double)
 
     @property
     def held(self) -> np.ndarray:
-        return np.ctypeslib.as_array(lib().orf_held(self._h), shape=(self.n,))
+        return self._view(lib().orf_held(self._h), C.c_uint8)
 
     @property
     def uptake(self) -> np.ndarray:
-        return np.ctypeslib.as_array(lib().orf_uptake(self._h), shape=(self.n,))
+        return self._view(lib().orf_uptake(self._h), C.c_double)
 
     @property
     def production(self) -> np.ndarray:
-        return np.ctypeslib.as_array(lib().orf_production(self._h), shape=(self.n,))
+        return self._view(lib().orf_production(self._h), C.c_double)
 
     def set_source(self, s):
         lib().orf_set_source(self._h, s)
@@ -204,7 +211,7 @@ class OracleField:
         return lib().orf_sor_sweep(self._h, omega, s, threads)
 
     def copy_next_from_values(self):
-        nxt = np.ctypeslib.as_array(lib().orf_next(self._h), shape=(self.n,))
+        nxt = self._view(lib().orf_next(self._h), C.c_double)
         nxt[:] = self.values
 
     def inflow(self):

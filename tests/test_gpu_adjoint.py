@@ -98,3 +98,41 @@ def test_adjoint_refuses_a_floored_field():
     f.relax_ex(omega=1.9, criterion=CRIT_REL_RESIDUAL, tolerance=1e-10, max_sweeps=200_000)
     with pytest.raises(ValueError, match="floor is active"):
         f.adjoint(np.ones(n * n))
+
+
+def test_spheroid_adjoint_matches_cpu_and_differences():
+    """The 192^3 adjoint workload scaled to 48^3: Voronoi spheroid with a
+    necrotic core, forward in FP32+refinement, adjoint in FP64, checked against
+    the CPU FP64 adjoint (1e-9) and central differences at h = 1e-6 relative
+    (1e-6), the perturbed solves in correction form."""
+    from cellweave_amd.field import PREC_FP32_REFINE
+    from cellweave_amd.workloads import SpheroidAdjoint
+
+    n = 48
+    s = n / 192
+    say(f"case spheroid adjoint {n}^3")
+    case = SpheroidAdjoint(n=n, ncells=int(25000 * s ** 3), radius=82 * s, necrotic=40 * s)
+    case.prepare()
+    rates = case.rates
+    f, _ = case.solve(case.fc.diff, 1.0, rates, precision=PREC_FP32_REFINE, tol=1e-12)
+    c = f.download()
+    assert c.min() > 0
+    assert np.max(np.abs(c - case.base)) <= 1e-10
+    got = f.adjoint(case.target, grid=case.fc.grid, ncells=rates.size, tolerance=1e-12,
+                    omega=case.omega)
+    ref = OracleField(n, n, n, case.fc.diff, 0.0, 1.0)
+    ref.values[:] = c
+    ref.uptake[:] = case.uptake_of(rates)
+    ref.held[:] = case.fc.held
+    want = ref.adjoint(case.target, omega=case.omega, rel_tol=1e-12, grid=case.fc.grid,
+                       ncells=rates.size)
+    assert abs(got["dl_dd"] - want["dD"]) <= 1e-9 * abs(want["dD"])
+    assert abs(got["dl_dg"] - want["dg"]) <= 1e-9 * abs(want["dg"])
+    assert np.max(np.abs(got["du"] - want["du"])) <= 1e-9 * np.max(np.abs(want["du"]))
+    live = np.flatnonzero(np.abs(got["du"]) > 0)
+    cells = np.random.default_rng(48).choice(live, size=4, replace=False)
+    fd = case.differences(h=1e-6, cells=cells)
+    assert abs(got["dl_dd"] - fd["D"]) <= 1e-6 * abs(fd["D"]), (got["dl_dd"], fd["D"])
+    assert abs(got["dl_dg"] - fd["g"]) <= 1e-6 * abs(fd["g"]), (got["dl_dg"], fd["g"])
+    for k, v in fd["cells"].items():
+        assert abs(got["du"][k] - v) <= 1e-6 * abs(v), (k, got["du"][k], v)

@@ -246,3 +246,21 @@ def test_adjoint_matches_finite_differences():
         dn[k] -= hk
         fd = (loss(solve(0.3, 1.0, up)) - loss(solve(0.3, 1.0, dn))) / (2 * hk)
         assert abs(adj["du"][k] - fd) <= 1e-5 * abs(fd), (k, adj["du"][k], fd)
+
+
+def test_views_keep_their_field_alive():
+    """A view taken from a temporary field stays valid after the temporary
+    is gone (the view holds the field)."""
+    import gc
+
+    def make():
+        f = OracleField(32, 32, 1, 0.3, 0.0, 0.0)
+        f.values[:] = np.arange(32 * 32, dtype=np.float64)
+        return f
+
+    v = make().values
+    gc.collect()
+    junk = [OracleField(32, 32, 1, 0.3, 0.0, 7.0) for _ in range(8)]
+    for j in junk:
+        j.values[:] = -1.0
+    assert np.array_equal(v, np.arange(32 * 32, dtype=np.float64))
