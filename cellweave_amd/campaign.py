@@ -65,18 +65,23 @@ def group(cases_):
     return groups
 
 
-def solve(fields, cases_, precision=PREC_FP32_REFINE, tol=1e-8, table=None, use_graph=1,
-          max_sweeps=2_000_000):
-    """Solve every field to `tol`; one batched launch sequence per size class.
-    Returns (stats per field, seconds)."""
+def plan(cases_, precision=PREC_FP32_REFINE, table=None):
+    """The launch plan, built before any timing: the fields of each size class
+    with the class's tile shape from the autotuner table."""
     table = load_table() if table is None else table
     key = "fp64" if precision == PREC_FP64 else "fp32"
+    return [(cls, idx, table.get(key, {}).get(cls, -1))
+            for cls, idx in sorted(group(cases_).items())]
+
+
+def solve(fields, plan_, precision=PREC_FP32_REFINE, tol=1e-8, use_graph=1,
+          max_sweeps=2_000_000, omega=-1.0):
+    """Solve every field to `tol`, one batched launch sequence per size class.
+    omega < 0 gives each field its own 2/(1+sin(pi/L)), L its longest free-span
+    edge. Returns (stats per field, seconds)."""
     stats = [None] * len(fields)
     t0 = time.perf_counter()
-    for cls, idx in sorted(group(cases_).items()):
-        shape = table.get(key, {}).get(cls, -1)
-        # one relaxation factor per class: the optimum for the class's largest box
-        omega = max(omega_for(cases_[k]) for k in idx)
+    for cls, idx, shape in plan_:
         st, _ = batch_relax([fields[k] for k in idx], criterion=CRIT_REL_RESIDUAL, tolerance=tol,
                             max_sweeps=max_sweeps, omega=omega, precision=precision,
                             tile_shape=shape, use_graph=use_graph)
@@ -91,10 +96,9 @@ def tune(cases_, per_class=8, launches=10):
     for cls, idx in sorted(group(cases_).items()):
         sample = [cases_[k] for k in idx[:per_class]]
         fields = make_fields(sample)
-        omega = max(omega_for(fc) for fc in sample)
         for key, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32_REFINE)):
             ranked = autotune(fields, launches=launches, criterion=CRIT_REL_RESIDUAL,
-                              tolerance=1e-8, omega=omega, precision=prec)
+                              tolerance=1e-8, omega=-1.0, precision=prec)
             table[key][cls] = ranked[0][0]
             table["ranking"][f"{key} {cls}"] = ranked[:6]
             for g, fc in zip(fields, sample):

@@ -221,6 +221,19 @@ double reference_omega(const cw_field* f) {
     return std::min(2.0 / (1.0 + std::sin(kPi / n)), 1.95);
 }
 
+// The model-problem optimum 2 / (1 + sin(pi / L)), L the longest edge of the
+// free-span box; the reference's volume rule, applied to planes too.
+double optimal_omega(const cw_field* f) {
+    int longest = 2;
+    for (int a = 0; a < 3; a++) longest = std::max(longest, f->span[2 * a + 1] - f->span[2 * a] + 1);
+    return 2.0 / (1.0 + std::sin(kPi / (double)longest));
+}
+
+double pick_omega(double asked, const cw_field* f) {
+    if (asked > 0.0) return asked;
+    return asked < 0.0 ? optimal_omega(f) : reference_omega(f);
+}
+
 // ---------------------------------------------------------------------------
 // Launch plan: problems, control blocks, tile tables, reduction scratch.
 // ---------------------------------------------------------------------------
@@ -798,7 +811,7 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
             stats[i].sweeps = o.crit == CRIT_MAXCHANGE ? 1 : 0;
             continue;
         }
-        omegas[i] = o.omega > 0.0 ? o.omega : reference_omega(f);
+        omegas[i] = pick_omega(o.omega, f);
         stats[i].omega = omegas[i];
         stats[i].span_voxels = span_voxels(f);
         if (o.max_sweeps == 0 && o.time_passes == 0) {
@@ -881,22 +894,50 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
 // clamped to [lo, hi]. While the residual has not yet started to fall (the
 // SOR transient) the segment doubles.
 int next_segment(const std::vector<double>& res, const std::vector<int>& at, double target,
-                 double per_segment, int first, int lo, int hi, int s) {
+                 double per_segment, int first, int lo, int hi, int s, double model_rate = 0.0) {
     int k = first;
     const size_t n = res.size();
     if (n >= 2) {
         const int last = at[n - 1] - at[n - 2];
+        const double goal = std::max(target / res[n - 1], per_segment);
         if (res[n - 1] > 0.0 && res[n - 2] > res[n - 1] && last > 0) {
             const double rate = std::log(res[n - 1] / res[n - 2]) / (double)last;
-            const double goal = std::max(target / res[n - 1], per_segment);
             const double need = std::log(goal) / rate;
             k = need > 0.0 ? (int)std::ceil(1.1 * need) + s : lo;
         } else {
             k = 2 * std::max(last, first);
         }
+        // A rate measured across the SOR transient can be far slower than the
+        // asymptotic one; the model problem's rate bounds the budget.
+        if (model_rate < 0.0 && goal < 1.0)
+            k = std::min(k, (int)std::ceil(1.5 * std::log(goal) / model_rate) + first / 2);
     }
     k = std::max(lo, std::min(hi, k));
     return (k + s - 1) / s * s;
+}
+
+// log of the asymptotic residual reduction per sweep of red-black SOR on the
+// field's free-span box with all faces held (the model problem): with mu the
+// Jacobi radius, rho = omega - 1 above the optimum and
+// ((omega mu + sqrt(omega^2 mu^2 - 4 (omega - 1))) / 2)^2 below it (Young's
+// theory for consistently ordered matrices). Decay shrinks mu by A_d / (A_d +
+// decay). Fields held on more than the box converge at least this fast.
+double model_log_rate(const cw_field* f, double omega) {
+    const int dims = f->D > 1 ? 3 : 2;
+    double mu = 0.0;
+    for (int a = 0; a < dims; a++)
+        mu += std::cos(kPi / (double)(f->span[2 * a + 1] - f->span[2 * a] + 2));
+    mu /= (double)dims;
+    const double ad = 2.0 * dims * f->diff;
+    if (ad + f->decay > 0.0) mu *= ad / (ad + f->decay);
+    const double disc = omega * omega * mu * mu - 4.0 * (omega - 1.0);
+    double rho = omega - 1.0;
+    if (disc > 0.0) {
+        const double r = 0.5 * (omega * mu + std::sqrt(disc));
+        rho = r * r;
+    }
+    rho = std::fabs(rho);
+    return (rho > 0.0 && rho < 1.0) ? std::log(rho) : 0.0;
 }
 
 // FP32 smoothing of the defect equation with FP64 residual and correction.
@@ -936,7 +977,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         stats[i].sweeps_per_pass = s;
         if (!f->has_span) continue;
         if ((st = ensure_fp32(f))) return st;
-        omegas[i] = o.omega > 0.0 ? o.omega : reference_omega(f);
+        omegas[i] = pick_omega(o.omega, f);
         stats[i].omega = omegas[i];
         stats[i].span_voxels = span_voxels(f);
         for (int a = 0; a < 3; a++) longest = std::max(longest, f->span[2 * a + 1] - f->span[2 * a] + 1);
@@ -946,7 +987,9 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         pr.k2 = f->general ? f->k2 : nullptr;
         pr.c64 = f->c[f->cur];
         pr.e0 = f->e[0];
-        pr.e1 = f->e[1];
+        // correct64 reads the inner answer from e[parity]; an in-place inner
+        // plan keeps it in e[0] whatever the parity
+        pr.e1 = (three_d && k3DShapes[inner.shape3].kind == 0) ? f->e[0] : f->e[1];
         pr.r32 = f->r32;
         pr.lb32 = f->has_sink ? f->lb : nullptr;
         pr.inner_idx = (int)idx.size();
@@ -977,6 +1020,8 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         if ((st = write_ctl(inner))) return st;
         return time_launches(inner, o.time_passes, o.time_ms);
     }
+    std::vector<double> model_rate(np);
+    for (int p = 0; p < np; p++) model_rate[p] = model_log_rate(F[idx[p]], outer.hp[p].omega);
     std::vector<std::vector<double>> hist(np);
     std::vector<std::vector<int>> hist_at(np);
     std::vector<int> total(np, 0), outer_it(np, 0);
@@ -996,7 +1041,8 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
             const bool run = outer.hc[p].status == ST_RUNNING;
             init_ctl(inner.hc[p], run ? ST_RUNNING : ST_IDLE);
             if (!run) continue;
-            int k = next_segment(hist[p], hist_at[p], o.tol, o.inner_tol, first, 16, hi, s);
+            int k = next_segment(hist[p], hist_at[p], o.tol, o.inner_tol, first, 16, hi, s,
+                                 model_rate[p]);
             if ((uint64_t)total[p] + k > o.max_sweeps)
                 k = (int)(o.max_sweeps - (uint64_t)total[p]);
             inner.hc[p].budget = std::max(k, 1);
@@ -1152,7 +1198,7 @@ cw_status field_adjoint(cw_field* f, const double* target, const uint32_t* grid,
                          f->stream) != hipSuccess)
         return bail(fail(CW_ERR_DEVICE, "adjoint target upload failed"));
     const bool three_d = f->D > 1;
-    const double omega = o.omega > 0.0 ? o.omega : reference_omega(f);
+    const double omega = pick_omega(o.omega, f);
     // smoother plan (lambda, pre-scaled beta) and residual plan (lambda, rhs)
     Plan sm, rs;
     sm.stream = rs.stream = f->stream;

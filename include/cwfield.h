@@ -57,7 +57,8 @@ typedef enum cw_precision {
 } cw_precision;
 
 typedef struct cw_relax_opts {
-    double omega;             /* <= 0: the reference rule (pde.rs:267-277) */
+    double omega;             /* 0: the reference rule (pde.rs:267-277); < 0: each field's
+                                 2/(1+sin(pi/L)), L its longest free-span edge */
     int32_t criterion;        /* cw_criterion */
     double tolerance;         /* absolute max change, or relative residual */
     uint64_t max_sweeps;      /* hard limit; reaching it is CW_ERR_NOT_CONVERGED */

@@ -279,3 +279,18 @@ def test_fp32_refine_3d(shape):
     chk.held[:] = ref.held
     assert chk.rel_residual()[0] <= 1e-8
     assert np.max(np.abs(v - ref.values)) < 1e-6
+
+
+@pytest.mark.parametrize("inner_tol", [0.1, 0.03])
+def test_fp32_refine_3d_in_place_odd_segments(inner_tol):
+    """The in-place volume smoother keeps its answer in one buffer, so the
+    correction must read it there whatever the parity of a segment's last
+    pass (budgets of odd length)."""
+    say(f"case 3d refine in place, inner_tol={inner_tol}")
+    ref, g = ball_case(side=29, r=11.0)
+    ref.relax_residual(1e-13, 1_000_000, omega=1.75)
+    st = g.relax_ex(omega=1.75, criterion=CRIT_REL_RESIDUAL, tolerance=1e-10, max_sweeps=50_000,
+                    precision=PREC_FP32_REFINE, tile_shape=0, inner_tolerance=inner_tol)
+    say(f"  refine stats {st}")
+    assert st["outer_iters"] >= 4
+    assert np.max(np.abs(g.download() - ref.values)) < 1e-8
