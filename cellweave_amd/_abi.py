@@ -97,6 +97,23 @@ def _declare():
         "cw_batch_time_passes": (st, [C.POINTER(P), u32, opts, u32, dp]),
         "cw_field_adjoint": (st, [P, dp, u32p, u32, opts, dp, C.POINTER(AdjointResult)]),
         "cw_autotune": (st, [C.POINTER(P), u32, opts, u32, C.POINTER(C.c_int32), dp, u32, u32p]),
+        "cw_lattice_create": (st, [u32, u32, u32, C.POINTER(P)]),
+        "cw_lattice_destroy": (st, [P]),
+        "cw_lattice_upload": (st, [P, u32p]),
+        "cw_field_apply_uptake": (st, [P, P, u8p, u32, d]),
+        "cw_field_apply_uptake_per_cell": (st, [P, P, u8p, dp, u32]),
+        "cw_field_apply_background_uptake": (st, [P, P, d]),
+        "cw_field_apply_production": (st, [P, P, dp, u32]),
+        "cw_field_hold_obstacles_at": (st, [P, P, d]),
+        "cw_field_hold_where": (st, [P, u8p, d]),
+        "cw_field_hold_bath_at": (st, [P, u8p, d]),
+        "cw_field_hold_medium_bath": (st, [P, P, d]),
+        "cw_field_release_all": (st, [P]),
+        "cw_field_free_count": (st, [P, u64p]),
+        "cw_field_inflow": (st, [P, dp]),
+        "cw_field_step": (st, [P, d]),
+        "cw_field_mean_per_cell": (st, [P, P, u32, dp]),
+        "cw_field_download_state": (st, [P, u8p, dp, dp]),
     })
 
 

@@ -5,9 +5,7 @@
 // ScalarField::relax (pde.rs:245-288), FP32 smoothing with FP64 defect
 // correction, and the batched solve. Kernels live in cw_kernels.hpp.
 #include "cw_kernels.hpp"
-#include "../../include/cwfield.h"
-
-#include <hip/hip_runtime.h>
+#include "cw_internal.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -21,9 +19,15 @@
 
 using namespace cw;
 
-namespace {
+namespace cwi {
 
+namespace {
 thread_local std::string g_err;
+std::once_flag g_init_once;
+cw_status g_init_status = CW_ERR_DEVICE;
+std::string g_device_name;
+std::string g_version;
+}  // namespace
 
 cw_status fail(cw_status code, const char* fmt, ...) {
     char buf[512];
@@ -34,19 +38,6 @@ cw_status fail(cw_status code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
-
-#define CW_HIP(expr)                                                                     \
-    do {                                                                                 \
-        hipError_t e_ = (expr);                                                          \
-        if (e_ != hipSuccess)                                                            \
-            return fail(CW_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                             \
-    } while (0)
-
-std::once_flag g_init_once;
-cw_status g_init_status = CW_ERR_DEVICE;
-std::string g_device_name;
-std::string g_version;
 
 cw_status device_init() {
     std::call_once(g_init_once, [] {
@@ -65,6 +56,26 @@ cw_status device_init() {
     });
     if (g_init_status != CW_OK) return fail(CW_ERR_DEVICE, "no usable HIP device (gfx950 expected)");
     return CW_OK;
+}
+
+cw_status alloc_fill(void** p, size_t bytes, hipStream_t s, int fill) {
+    CW_HIP(hipMalloc(p, bytes));
+    CW_HIP(hipMemsetAsync(*p, fill, bytes, s));
+    return CW_OK;
+}
+
+const char* last_error_text() { return g_err.c_str(); }
+const std::string& device_name() { return g_device_name; }
+std::string& version_text() { return g_version; }
+
+}  // namespace cwi
+
+using namespace cwi;
+
+namespace {
+
+cw_status alloc_zero(void** p, size_t bytes, hipStream_t s, int fill) {
+    return alloc_fill(p, bytes, s, fill);
 }
 
 constexpr double kPi = 3.14159265358979323846;
@@ -97,107 +108,7 @@ int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 
 
 }  // namespace
 
-// ---------------------------------------------------------------------------
-// Field handle.
-// ---------------------------------------------------------------------------
-struct cw_field {
-    uint32_t W = 0, H = 0, D = 0;
-    int P = 0;
-    size_t n = 0, npad = 0;
-    double diff = 0, decay = 0, source = 0;
-    hipStream_t stream = nullptr;
-    double* c[2] = {nullptr, nullptr};
-    int cur = 0;
-    double* k = nullptr;
-    double* k2 = nullptr;   // uptake, general form only
-    bool general = false;
-    float* e[2] = {nullptr, nullptr};
-    float* r32 = nullptr;
-    float* lb = nullptr;
-    std::vector<uint8_t> held;
-    std::vector<double> uptake, production;
-    bool has_span = false;
-    int span[6] = {0, 0, 0, 0, 0, 0};
-    bool has_sink = false;
-    bool coef_dirty = true;
-};
-
 namespace {
-
-cw_status alloc_zero(void** p, size_t bytes, hipStream_t s, int fill) {
-    CW_HIP(hipMalloc(p, bytes));
-    CW_HIP(hipMemsetAsync(*p, fill, bytes, s));
-    return CW_OK;
-}
-
-void compute_span(cw_field* f) {
-    const size_t w = f->W, h = f->H;
-    bool found = false;
-    for (size_t i = 0; i < f->n; i++) {
-        if (f->held[i]) continue;
-        const int x = (int)(i % w), y = (int)((i / w) % h), z = (int)(i / (w * h));
-        if (!found) {
-            f->span[0] = f->span[1] = x;
-            f->span[2] = f->span[3] = y;
-            f->span[4] = f->span[5] = z;
-            found = true;
-        }
-        f->span[0] = std::min(f->span[0], x);
-        f->span[1] = std::max(f->span[1], x);
-        f->span[2] = std::min(f->span[2], y);
-        f->span[3] = std::max(f->span[3], y);
-        f->span[5] = z;
-    }
-    f->has_span = found;
-}
-
-// Coefficient arrays from held/uptake/production. The folded form k = p - u
-// (p when u == 0, -u when p == 0) reproduces the reference's
-// ((d*sum + source) + p) - u bit for bit when source == 0 and no voxel has
-// both; otherwise k = p and k2 = u are kept apart (general form).
-cw_status build_coefficients(cw_field* f) {
-    bool general = f->source != 0.0;
-    bool sink = false;
-    for (size_t i = 0; i < f->n && !general; i++)
-        if (!f->held[i] && f->production[i] != 0.0 && f->uptake[i] != 0.0) general = true;
-    const double qnan = std::nan("");
-    std::vector<double> k(f->n), k2;
-    if (general) k2.resize(f->n);
-    for (size_t i = 0; i < f->n; i++) {
-        if (f->held[i]) {
-            k[i] = qnan;
-            if (general) k2[i] = 0.0;
-            continue;
-        }
-        const double p = f->production[i], u = f->uptake[i];
-        if (general) {
-            k[i] = p;
-            k2[i] = u;
-            if ((f->source + p) - u < 0.0) sink = true;
-        } else {
-            k[i] = (p != 0.0) ? p : -u;
-            if (k[i] < 0.0) sink = true;
-        }
-    }
-    const size_t pitch = (size_t)f->P * sizeof(double);
-    if (general && !f->k2) {
-        cw_status st = alloc_zero((void**)&f->k2, f->npad * sizeof(double), f->stream, 0);
-        if (st) return st;
-    }
-    CW_HIP(hipMemsetAsync(f->k, 0xFF, f->npad * sizeof(double), f->stream));
-    CW_HIP(hipMemcpy2DAsync(f->k, pitch, k.data(), f->W * sizeof(double), f->W * sizeof(double),
-                            (size_t)f->H * f->D, hipMemcpyHostToDevice, f->stream));
-    if (general)
-        CW_HIP(hipMemcpy2DAsync(f->k2, pitch, k2.data(), f->W * sizeof(double),
-                                f->W * sizeof(double), (size_t)f->H * f->D,
-                                hipMemcpyHostToDevice, f->stream));
-    if (f->r32) CW_HIP(hipMemsetAsync(f->r32, 0xFF, f->npad * sizeof(float), f->stream));
-    CW_HIP(hipStreamSynchronize(f->stream));
-    f->general = general;
-    f->has_sink = sink;
-    f->coef_dirty = false;
-    return CW_OK;
-}
 
 cw_status ensure_fp32(cw_field* f) {
     if (f->r32) return CW_OK;
@@ -1157,12 +1068,19 @@ cw_status field_adjoint(cw_field* f, const double* target, const uint32_t* grid,
                             f->W * sizeof(double), (size_t)f->H * f->D, hipMemcpyDeviceToHost,
                             f->stream));
     CW_HIP(hipStreamSynchronize(f->stream));
+    std::vector<uint8_t> held(f->n);
+    std::vector<double> uptake;
+    if ((st = download_held(f, held.data()))) return st;
+    if (dl_du && ncells) {
+        uptake.resize(f->n);
+        if ((st = download_f64(f, f->up, uptake.data()))) return st;
+    }
     double loss = 0.0, dg_held = 0.0;
     size_t floored = 0;
     for (size_t i = 0; i < f->n; i++) {
         const double r = c[i] - target[i];
         loss += r * r;
-        if (f->held[i]) dg_held += r;
+        if (held[i]) dg_held += r;
         else if (!(c[i] > 0.0)) floored++;
     }
     res->loss = 0.5 * loss;
@@ -1292,7 +1210,7 @@ cw_status field_adjoint(cw_field* f, const double* target, const uint32_t* grid,
         const size_t W = f->W, H = f->H;
         for (size_t i = 0; i < f->n; i++) {
             const uint32_t id = grid[i];
-            if (f->held[i] || !(f->uptake[i] > 0.0) || id == 0 || id == 0xFFFFFFFFu || id >= ncells)
+            if (held[i] || !(uptake[i] > 0.0) || id == 0 || id == kObstacle || id >= ncells)
                 continue;
             off[id + 1]++;
         }
@@ -1301,7 +1219,7 @@ cw_status field_adjoint(cw_field* f, const double* target, const uint32_t* grid,
         std::vector<long long> fillp(off.begin(), off.end() - 1);
         for (size_t i = 0; i < f->n; i++) {
             const uint32_t id = grid[i];
-            if (f->held[i] || !(f->uptake[i] > 0.0) || id == 0 || id == 0xFFFFFFFFu || id >= ncells)
+            if (held[i] || !(uptake[i] > 0.0) || id == 0 || id == kObstacle || id >= ncells)
                 continue;
             const size_t x = i % W, y = (i / W) % H, z = i / (W * H);
             idx[(size_t)fillp[id]++] = (long long)((z * H + y) * (size_t)f->P + x);
@@ -1361,14 +1279,15 @@ void cw_relax_opts_default(cw_relax_opts* o) {
     o->use_graph = 0;
 }
 
-const char* cw_last_error(void) { return g_err.c_str(); }
+const char* cw_last_error(void) { return last_error_text(); }
 
 const char* cw_version(void) {
-    if (g_version.empty()) {
-        g_version = "cwfield 0.1.0 gfx950";
-        if (device_init() == CW_OK) g_version += " on " + g_device_name;
+    std::string& v = version_text();
+    if (v.empty()) {
+        v = "cwfield 0.2.0 gfx950";
+        if (device_init() == CW_OK) v += " on " + device_name();
     }
-    return g_version.c_str();
+    return v.c_str();
 }
 
 cw_status cw_field_create(uint32_t w, uint32_t h, uint32_t d, double diff, double decay,
@@ -1390,23 +1309,27 @@ cw_status cw_field_create(uint32_t w, uint32_t h, uint32_t d, double diff, doubl
     f->n = (size_t)w * h * d;
     f->npad = (size_t)f->P * h * d;
     f->diff = diff; f->decay = decay;
-    f->held.assign(f->n, 0);
-    f->uptake.assign(f->n, 0.0);
-    f->production.assign(f->n, 0.0);
     auto bail = [&](cw_status s) { cw_field_destroy(f); return s; };
     if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(CW_ERR_DEVICE, "hipStreamCreate failed"));
-    if ((st = alloc_zero((void**)&f->c[0], f->npad * sizeof(double), f->stream, 0))) return bail(st);
-    if ((st = alloc_zero((void**)&f->c[1], f->npad * sizeof(double), f->stream, 0))) return bail(st);
-    if ((st = alloc_zero((void**)&f->k, f->npad * sizeof(double), f->stream, 0xFF))) return bail(st);
+    const size_t b8 = f->npad * sizeof(double);
+    if ((st = alloc_zero((void**)&f->c[0], b8, f->stream, 0))) return bail(st);
+    if ((st = alloc_zero((void**)&f->c[1], b8, f->stream, 0))) return bail(st);
+    if ((st = alloc_zero((void**)&f->k, b8, f->stream, 0xFF))) return bail(st);
+    if ((st = alloc_zero((void**)&f->up, b8, f->stream, 0))) return bail(st);
+    if ((st = alloc_zero((void**)&f->pr, b8, f->stream, 0))) return bail(st);
+    if ((st = alloc_zero((void**)&f->held, f->npad, f->stream, 1))) return bail(st);
+    if ((st = alloc_zero((void**)&f->scan, 64 * sizeof(int), f->stream, 0))) return bail(st);
+    // nothing held inside the box; the padding stays held
+    CW_HIP(hipMemset2DAsync(f->held, f->P, 0, f->W, (size_t)f->H * f->D, f->stream));
     std::vector<double> init(f->n, initial);
     if (hipMemcpy2DAsync(f->c[0], f->P * sizeof(double), init.data(), w * sizeof(double),
                          w * sizeof(double), (size_t)h * d, hipMemcpyHostToDevice,
                          f->stream) != hipSuccess ||
         hipStreamSynchronize(f->stream) != hipSuccess)
         return bail(fail(CW_ERR_DEVICE, "initial upload failed"));
-    compute_span(f);
     f->coef_dirty = true;
+    if ((st = build_coefficients(f))) return bail(st);
     *out = f;
     return CW_OK;
 }
@@ -1415,6 +1338,7 @@ cw_status cw_field_destroy(cw_field* f) {
     if (!f) return CW_OK;
     if (f->stream) hipStreamSynchronize(f->stream);
     hipFree(f->c[0]); hipFree(f->c[1]); hipFree(f->k); hipFree(f->k2);
+    hipFree(f->up); hipFree(f->pr); hipFree(f->held); hipFree(f->scan);
     hipFree(f->e[0]); hipFree(f->e[1]); hipFree(f->r32); hipFree(f->lb);
     if (f->stream) hipStreamDestroy(f->stream);
     delete f;
@@ -1431,19 +1355,27 @@ cw_status cw_field_set_source(cw_field* f, double source) {
 cw_status cw_field_upload(cw_field* f, const double* values, const uint8_t* held,
                           const double* uptake, const double* production) {
     if (!f) return fail(CW_ERR_ARG, "null field");
+    const size_t pitch8 = (size_t)f->P * sizeof(double), row8 = f->W * sizeof(double);
+    const size_t rows = (size_t)f->H * f->D;
     if (held) {
-        for (size_t i = 0; i < f->n; i++) f->held[i] = held[i] ? 1 : 0;
-        compute_span(f);
+        std::vector<uint8_t> h(f->n);
+        for (size_t i = 0; i < f->n; i++) h[i] = held[i] ? 1 : 0;
+        CW_HIP(hipMemcpy2DAsync(f->held, f->P, h.data(), f->W, f->W, rows, hipMemcpyHostToDevice,
+                                f->stream));
+        CW_HIP(hipStreamSynchronize(f->stream));
         f->coef_dirty = true;
     }
     // The setters clamp at zero (pde.rs:146-182); NaN clamps to zero as
     // f64::max(NaN, 0.0) does.
-    if (uptake) {
-        for (size_t i = 0; i < f->n; i++) f->uptake[i] = uptake[i] > 0.0 ? uptake[i] : 0.0;
-        f->coef_dirty = true;
-    }
-    if (production) {
-        for (size_t i = 0; i < f->n; i++) f->production[i] = production[i] > 0.0 ? production[i] : 0.0;
+    std::vector<double> tmp;
+    for (int which = 0; which < 2; which++) {
+        const double* src = which ? production : uptake;
+        if (!src) continue;
+        tmp.resize(f->n);
+        for (size_t i = 0; i < f->n; i++) tmp[i] = src[i] > 0.0 ? src[i] : 0.0;
+        CW_HIP(hipMemcpy2DAsync(which ? f->pr : f->up, pitch8, tmp.data(), row8, row8, rows,
+                                hipMemcpyHostToDevice, f->stream));
+        CW_HIP(hipStreamSynchronize(f->stream));
         f->coef_dirty = true;
     }
     if (values) {

@@ -1,8 +1,10 @@
 """Python handle on the GPU field solve (thin layer over the C ABI).
 
-Field mirrors the device side of cellweave's ScalarField
-(crates/engine/src/pde.rs:30-53): the caller owns the host arrays and uploads
-them in bulk, relax() runs on the GPU, download() brings the values back.
+Field mirrors cellweave's ScalarField (crates/engine/src/pde.rs:30-53) with
+its state on the device: bulk upload, or the gathers from a Lattice (the
+cell-id lattice, crates/engine/src/cpm.rs:198) as crates/engine/src/coupling.rs
+does them; relax() and step() run on the GPU, download() brings the values
+back.
 Errors map onto the reference's two variants: ConfigError for
 CellweaveError::Config and SimulationError for CellweaveError::Simulation
 (crates/core/src/error.rs:7-14).
@@ -72,6 +74,39 @@ def make_opts(omega=0.0, criterion=CRIT_MAX_CHANGE, tolerance=1e-8, max_sweeps=2
     return o
 
 
+def _u8(a, n):
+    a = np.ascontiguousarray(a, dtype=np.uint8).reshape(-1)
+    assert a.size == n, (a.size, n)
+    return a
+
+
+class Lattice:
+    """Device copy of CpmLattice::grid (cpm.rs:198): u32 cell ids, 0 the
+    medium, 0xFFFFFFFF an obstacle (cpm.rs:137)."""
+
+    def __init__(self, width, height, depth=1):
+        self._h = C.c_void_p()
+        _check(_abi.lib().cw_lattice_create(width, height, depth, C.byref(self._h)))
+        self.w, self.h, self.d = width, height, depth
+        self.n = width * height * depth
+
+    def upload(self, ids):
+        a = np.ascontiguousarray(ids, dtype=np.uint32).reshape(-1)
+        assert a.size == self.n
+        _check(_abi.lib().cw_lattice_upload(self._h, a.ctypes.data_as(C.POINTER(C.c_uint32))))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _abi.lib().cw_lattice_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Field:
     def __init__(self, width, height, depth=1, diff=0.2, decay=0.0, initial=0.0):
         L = _abi.lib()
@@ -121,6 +156,87 @@ class Field:
         out = np.empty(self.n, dtype=np.float64)
         _check(_abi.lib().cw_field_download(self._h, _dptr(out)))
         return out
+
+    # ---- state on the device: gathers, holds, step (coupling.rs, pde.rs) ----
+    def apply_uptake(self, lattice: Lattice, consumes, rate: float):
+        """coupling.rs:26-39; consumes[id] != 0 for a living cell kind."""
+        c = _u8(consumes, len(consumes))
+        _check(_abi.lib().cw_field_apply_uptake(self._h, lattice._h,
+                                                c.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                c.size, rate))
+
+    def apply_uptake_per_cell(self, lattice: Lattice, rates, consumes=None):
+        r = np.ascontiguousarray(rates, dtype=np.float64).reshape(-1)
+        cp = None
+        if consumes is not None:
+            c = _u8(consumes, r.size)
+            cp = c.ctypes.data_as(C.POINTER(C.c_uint8))
+        _check(_abi.lib().cw_field_apply_uptake_per_cell(self._h, lattice._h, cp, _dptr(r),
+                                                         r.size))
+
+    def apply_background_uptake(self, lattice: Lattice, rate: float):
+        """coupling.rs:47-53."""
+        _check(_abi.lib().cw_field_apply_background_uptake(self._h, lattice._h, rate))
+
+    def apply_production(self, lattice: Lattice, per_cell):
+        """coupling.rs:60-73."""
+        r = np.ascontiguousarray(per_cell, dtype=np.float64).reshape(-1)
+        _check(_abi.lib().cw_field_apply_production(self._h, lattice._h, _dptr(r), r.size))
+
+    def hold_obstacles_at(self, lattice: Lattice, value: float):
+        """coupling.rs:79-81."""
+        _check(_abi.lib().cw_field_hold_obstacles_at(self._h, lattice._h, value))
+
+    def hold_where(self, mask, value: float):
+        """pde.rs:491-498 with the predicate as a mask."""
+        m = _u8(mask, self.n)
+        _check(_abi.lib().cw_field_hold_where(self._h, m.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                              value))
+
+    def hold_bath_at(self, open_mask, value: float):
+        """pde.rs:550-631 with is_open as a mask."""
+        m = _u8(open_mask, self.n)
+        _check(_abi.lib().cw_field_hold_bath_at(self._h, m.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                value))
+
+    def hold_medium_bath(self, lattice: Lattice, value: float):
+        """coupling.rs:148-150."""
+        _check(_abi.lib().cw_field_hold_medium_bath(self._h, lattice._h, value))
+
+    def release_all(self):
+        """pde.rs:465-467."""
+        _check(_abi.lib().cw_field_release_all(self._h))
+
+    def free_count(self) -> int:
+        """pde.rs:458-461."""
+        v = C.c_uint64()
+        _check(_abi.lib().cw_field_free_count(self._h, C.byref(v)))
+        return int(v.value)
+
+    def inflow(self) -> float:
+        """pde.rs:415-456 (fixed-tree sum)."""
+        v = C.c_double()
+        _check(_abi.lib().cw_field_inflow(self._h, C.byref(v)))
+        return v.value
+
+    def step(self, dt: float):
+        """DiffusionField::step (pde.rs:705-728)."""
+        _check(_abi.lib().cw_field_step(self._h, dt))
+
+    def mean_per_cell(self, lattice: Lattice, ncells: int) -> np.ndarray:
+        """coupling.rs:164-182."""
+        out = np.zeros(max(int(ncells), 1), dtype=np.float64)
+        _check(_abi.lib().cw_field_mean_per_cell(self._h, lattice._h, int(ncells), _dptr(out)))
+        return out[:ncells]
+
+    def state(self):
+        """(held, uptake, production) as host arrays."""
+        held = np.empty(self.n, dtype=np.uint8)
+        up = np.empty(self.n, dtype=np.float64)
+        pr = np.empty(self.n, dtype=np.float64)
+        _check(_abi.lib().cw_field_download_state(self._h, held.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                  _dptr(up), _dptr(pr)))
+        return held, up, pr
 
     def relax(self, tolerance: float, max_sweeps: int) -> int:
         """ScalarField::relax (pde.rs:245-288): returns the sweep count."""

@@ -3,9 +3,11 @@
  *
  * This is the boundary a thin FFI crate binds so that cellweave's
  * ScalarField (crates/engine/src/pde.rs:30-53) can delegate its steady-state
- * solve to the GPU without changing any Rust signature. The host keeps the
- * canonical arrays; a cw_field handle owns their device mirrors, the
- * ping-pong state buffers, the reduction scratch and one HIP stream.
+ * solve to the GPU without changing any Rust signature. A cw_field handle
+ * owns the field's state on the device (values, held, uptake, production),
+ * the ping-pong buffers, the reduction scratch and one HIP stream; the host
+ * fills it by bulk upload or by the device-side gathers from a cw_lattice
+ * (the cell-id lattice of crates/engine/src/cpm.rs:198).
  *
  * Plain C types only: pointers, sizes, doubles. Every entry point returns a
  * cw_status; on failure cw_last_error() gives a thread-local message.
@@ -26,6 +28,7 @@ extern "C" {
 #endif
 
 typedef struct cw_field cw_field;
+typedef struct cw_lattice cw_lattice;
 
 /* Error codes. CW_ERR_CONFIG maps onto CellweaveError::Config and
  * CW_ERR_NOT_CONVERGED onto CellweaveError::Simulation
@@ -183,6 +186,72 @@ cw_status cw_autotune(cw_field *const *fields, uint32_t count, const cw_relax_op
 cw_status cw_batch_time_passes(cw_field *const *fields, uint32_t count,
                                const cw_relax_opts *opts, uint32_t passes,
                                double *ms_per_pass);
+
+/* ---- the field's state on the device: lattice gathers, holds, step ---------
+ * Each call below is synchronous, like the Rust function it replaces, and
+ * marks the coefficients the next solve derives from the state as stale. */
+
+/* A device copy of CpmLattice::grid (cpm.rs:198): u32 cell ids, 0 = medium,
+ * 0xFFFFFFFF = obstacle (cpm.rs:137). Upload after every lattice change; the
+ * gathers below read it on the device instead of one FFI call per voxel. */
+cw_status cw_lattice_create(uint32_t width, uint32_t height, uint32_t depth, cw_lattice **out);
+cw_status cw_lattice_destroy(cw_lattice *lattice);
+cw_status cw_lattice_upload(cw_lattice *lattice, const uint32_t *ids);
+
+/* coupling.rs:26-39 apply_uptake: clear the sinks, then every voxel of a
+ * cell id < ncells with consumes[id] != 0 (coupling.rs:15-20, a living kind)
+ * takes up max(rate, 0). */
+cw_status cw_field_apply_uptake(cw_field *field, cw_lattice *lattice, const uint8_t *consumes,
+                                uint32_t ncells, double rate);
+/* apply_uptake with one rate per cell (the per-cell uptake vector the adjoint
+ * differentiates; no single counterpart in the reference): a voxel of cell id
+ * takes up rates[id] where that is positive. consumes may be NULL (all). */
+cw_status cw_field_apply_uptake_per_cell(cw_field *field, cw_lattice *lattice,
+                                         const uint8_t *consumes, const double *rates,
+                                         uint32_t ncells);
+/* coupling.rs:47-53 apply_background_uptake: every medium voxel (id 0) takes
+ * up max(rate, 0); other voxels keep their sink. */
+cw_status cw_field_apply_background_uptake(cw_field *field, cw_lattice *lattice, double rate);
+/* coupling.rs:60-73 apply_production: clear the sources, then every voxel of
+ * a cell id < ncells secretes per_cell[id] where that is positive. */
+cw_status cw_field_apply_production(cw_field *field, cw_lattice *lattice, const double *per_cell,
+                                    uint32_t ncells);
+
+/* coupling.rs:79-81 hold_obstacles_at: hold every obstacle voxel at value. */
+cw_status cw_field_hold_obstacles_at(cw_field *field, cw_lattice *lattice, double value);
+/* pde.rs:491-498 hold_where with the predicate as a byte mask (nonzero =
+ * hold): held, and the value set. */
+cw_status cw_field_hold_where(cw_field *field, const uint8_t *mask, double value);
+/* pde.rs:550-631 hold_bath_at with is_open as a byte mask (nonzero = open):
+ * the bath floods from the outer layer of the box around the closed voxels
+ * (one voxel wider) through the faces of open voxels not already held;
+ * everything outside that box and everything reached is held, and then every
+ * held voxel takes `value`. */
+cw_status cw_field_hold_bath_at(cw_field *field, const uint8_t *open, double value);
+/* coupling.rs:148-150 hold_medium_bath: hold_bath_at with open = medium (id 0). */
+cw_status cw_field_hold_medium_bath(cw_field *field, cw_lattice *lattice, double value);
+/* pde.rs:465-467 release_all. */
+cw_status cw_field_release_all(cw_field *field);
+
+/* pde.rs:458-461 free_count: voxels not held. */
+cw_status cw_field_free_count(cw_field *field, uint64_t *count);
+/* pde.rs:415-456 inflow: the flux D (c_j - c_i) over every face between a
+ * free voxel i and a held voxel j. Summed in a fixed tree (the same bits on
+ * every run); the reference's sequential order differs by rounding only. */
+cw_status cw_field_inflow(cw_field *field, double *flux);
+/* DiffusionField::step (pde.rs:705-728, laplacian 651-667, check_step
+ * 673-701): one explicit Euler step with no-flux walls, floored at zero;
+ * CW_ERR_CONFIG for a step the scheme cannot take. */
+cw_status cw_field_step(cw_field *field, double dt);
+/* coupling.rs:164-182 mean_per_cell: out[k] for k < ncells is the mean value
+ * over the voxels of cell k (0 for the medium and for a cell with none),
+ * added in increasing voxel order as the reference adds them. */
+cw_status cw_field_mean_per_cell(cw_field *field, cw_lattice *lattice, uint32_t ncells,
+                                 double *out);
+/* Host copies (n entries each; any pointer may be NULL) of the held mask,
+ * uptake and production, for the Rust getters (pde.rs:184-204). */
+cw_status cw_field_download_state(cw_field *field, uint8_t *held, double *uptake,
+                                  double *production);
 
 #ifdef __cplusplus
 }
