@@ -40,6 +40,8 @@ struct cw_lattice {
     size_t range_cap = 0;
     void* sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
+    void* stage = nullptr;
+    size_t stage_bytes = 0;
 };
 
 namespace {
@@ -83,8 +85,8 @@ __device__ inline int wave_or(int v) {
 }
 
 // ---- coefficient scan and build -------------------------------------------
-// scan[0] general, [1] folded sink, [2] general sink, [3] any free,
-// [4..9] free-span box (min x, max x, min y, max y, min z, max z).
+// scan[10] flag bits (1 general, 2 folded sink, 4 general sink, 8 any free),
+// scan[4..9] the free-span box (min x, max x, min y, max y, min z, max z).
 __global__ void k_scan_init(int* scan) {
     if (threadIdx.x < 16) {
         const int t = threadIdx.x;
@@ -92,36 +94,61 @@ __global__ void k_scan_init(int* scan) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan(Geo g, const uint8_t* held, const double* up,
-                                                 const double* pr, double source, int* scan) {
-    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    int x = 0, y = 0, z = 0;
-    bool free_ = false;
-    int gen = 0, sinkf = 0, sinkg = 0;
-    if (i < g.npad) {
-        coords(g, i, x, y, z);
-        if (x < g.W && !held[i]) {
-            free_ = true;
-            const double p = pr[i], u = up[i];
-            gen = (p != 0.0 && u != 0.0);
-            sinkf = (p != 0.0 ? p : -u) < 0.0;
-            sinkg = (source + p) - u < 0.0;
+constexpr int kRedBlocks = 2048;  // grid of the reductions: grid-stride, one atomic per block
+
+unsigned red_grid(size_t npad) {
+    return (unsigned)std::min<size_t>(kRedBlocks, (npad + kBlock - 1) / kBlock);
+}
+
+// Block-wide combine of (flags, box) and one set of atomics per block.
+__device__ inline void box_commit(int flags, int mnx, int mxx, int mny, int mxy, int mnz, int mxz,
+                                  int* out_flags, int* out_box) {
+    __shared__ int sh[kBlock / 64][7];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    flags = wave_or(flags);
+    mnx = wave_min(mnx); mxx = wave_max(mxx);
+    mny = wave_min(mny); mxy = wave_max(mxy);
+    mnz = wave_min(mnz); mxz = wave_max(mxz);
+    if (lane == 0) {
+        sh[wv][0] = flags; sh[wv][1] = mnx; sh[wv][2] = mxx; sh[wv][3] = mny;
+        sh[wv][4] = mxy; sh[wv][5] = mnz; sh[wv][6] = mxz;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; w++) {
+            flags |= sh[w][0];
+            mnx = min(mnx, sh[w][1]); mxx = max(mxx, sh[w][2]);
+            mny = min(mny, sh[w][3]); mxy = max(mxy, sh[w][4]);
+            mnz = min(mnz, sh[w][5]); mxz = max(mxz, sh[w][6]);
+        }
+        if (flags) {
+            atomicOr(out_flags, flags);
+            if (mxx >= 0) {
+                atomicMin(&out_box[0], mnx); atomicMax(&out_box[1], mxx);
+                atomicMin(&out_box[2], mny); atomicMax(&out_box[3], mxy);
+                atomicMin(&out_box[4], mnz); atomicMax(&out_box[5], mxz);
+            }
         }
     }
-    const int lane = threadIdx.x & 63;
-    const int a = wave_or(gen | (sinkf << 1) | (sinkg << 2) | ((int)free_ << 3));
-    const int mnx = wave_min(free_ ? x : INT_MAX), mxx = wave_max(free_ ? x : -1);
-    const int mny = wave_min(free_ ? y : INT_MAX), mxy = wave_max(free_ ? y : -1);
-    const int mnz = wave_min(free_ ? z : INT_MAX), mxz = wave_max(free_ ? z : -1);
-    if (lane == 0 && (a & 8)) {
-        if (a & 1) atomicOr(&scan[0], 1);
-        if (a & 2) atomicOr(&scan[1], 1);
-        if (a & 4) atomicOr(&scan[2], 1);
-        atomicOr(&scan[3], 1);
-        atomicMin(&scan[4], mnx); atomicMax(&scan[5], mxx);
-        atomicMin(&scan[6], mny); atomicMax(&scan[7], mxy);
-        atomicMin(&scan[8], mnz); atomicMax(&scan[9], mxz);
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan(Geo g, const uint8_t* held, const double* up,
+                                                 const double* pr, double source, int* scan) {
+    int flags = 0;
+    int mnx = INT_MAX, mxx = -1, mny = INT_MAX, mxy = -1, mnz = INT_MAX, mxz = -1;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < g.npad;
+         i += (size_t)gridDim.x * kBlock) {
+        int x, y, z;
+        coords(g, i, x, y, z);
+        if (x >= g.W || held[i]) continue;
+        const double p = pr[i], u = up[i];
+        flags |= (p != 0.0 && u != 0.0) | (((p != 0.0 ? p : -u) < 0.0) << 1) |
+                 ((((source + p) - u) < 0.0) << 2) | 8;
+        mnx = min(mnx, x); mxx = max(mxx, x);
+        mny = min(mny, y); mxy = max(mxy, y);
+        mnz = min(mnz, z); mxz = max(mxz, z);
     }
+    box_commit(flags, mnx, mxx, mny, mxy, mnz, mxz, &scan[10], &scan[4]);
 }
 
 // k = p or -u (folded, exact when no voxel has both and source == 0) or
@@ -244,23 +271,19 @@ __global__ void k_box_init(int* scan) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_closed_box(Geo g, const uint8_t* open, int* scan) {
-    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    int x = 0, y = 0, z = 0;
-    bool closed = false;
-    if (i < g.npad) {
+    int flags = 0;
+    int mnx = INT_MAX, mxx = -1, mny = INT_MAX, mxy = -1, mnz = INT_MAX, mxz = -1;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < g.npad;
+         i += (size_t)gridDim.x * kBlock) {
+        int x, y, z;
         coords(g, i, x, y, z);
-        closed = x < g.W && !open[i];
+        if (x >= g.W || open[i]) continue;
+        flags = 1;
+        mnx = min(mnx, x); mxx = max(mxx, x);
+        mny = min(mny, y); mxy = max(mxy, y);
+        mnz = min(mnz, z); mxz = max(mxz, z);
     }
-    const int any = wave_or((int)closed);
-    const int mnx = wave_min(closed ? x : INT_MAX), mxx = wave_max(closed ? x : -1);
-    const int mny = wave_min(closed ? y : INT_MAX), mxy = wave_max(closed ? y : -1);
-    const int mnz = wave_min(closed ? z : INT_MAX), mxz = wave_max(closed ? z : -1);
-    if ((threadIdx.x & 63) == 0 && any) {
-        atomicOr(&scan[0], 1);
-        atomicMin(&scan[1], mnx); atomicMax(&scan[2], mxx);
-        atomicMin(&scan[3], mny); atomicMax(&scan[4], mxy);
-        atomicMin(&scan[5], mnz); atomicMax(&scan[6], mxz);
-    }
+    box_commit(flags, mnx, mxx, mny, mxy, mnz, mxz, &scan[0], &scan[1]);
 }
 
 // Seeds: open, not yet held voxels on the outer layer of the box (its first
@@ -368,15 +391,92 @@ __global__ __launch_bounds__(kBlock) void k_flood_finish(Geo g, Box b, int has_b
 // ---- reductions: free count and inflow (pde.rs:415-461) ----------------------
 __global__ __launch_bounds__(kBlock) void k_count_free(Geo g, const uint8_t* held,
                                                        unsigned long long* out) {
-    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
-    int x = 0, y = 0, z = 0;
+    __shared__ unsigned long long sh[kBlock / 64];
     unsigned long long v = 0;
-    if (i < g.npad) {
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < g.npad;
+         i += (size_t)gridDim.x * kBlock) {
+        int x, y, z;
         coords(g, i, x, y, z);
-        v = (x < g.W && !held[i]) ? 1 : 0;
+        v += (x < g.W && !held[i]) ? 1 : 0;
     }
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; w++) v += sh[w];
+        if (v) atomicAdd(out, v);
+    }
+}
+
+// ---- host <-> device copies through a flat staging buffer ---------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_unflatten(const T* flat, T* pitched, int W,
+                                                      size_t rows, int P) {
+    const size_t j = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= (size_t)W * rows) return;
+    const size_t row = j / (size_t)W, x = j - row * (size_t)W;
+    pitched[row * (size_t)P + x] = flat[j];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_flatten(const T* pitched, T* flat, int W, size_t rows,
+                                                    int P) {
+    const size_t j = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= (size_t)W * rows) return;
+    const size_t row = j / (size_t)W, x = j - row * (size_t)W;
+    flat[j] = pitched[row * (size_t)P + x];
+}
+
+cw_status stage_ensure(void** stage, size_t* cap, size_t bytes) {
+    if (*cap >= bytes) return CW_OK;
+    (void)hipFree(*stage);
+    *stage = nullptr;
+    *cap = 0;
+    CW_HIP(hipMalloc(stage, bytes));
+    *cap = bytes;
+    return CW_OK;
+}
+
+cw_status copy_in(hipStream_t s, int W, size_t rows, int P, void* dst, const void* src, int elem,
+                  void** stage, size_t* cap) {
+    const size_t n = (size_t)W * rows;
+    cw_status st = stage_ensure(stage, cap, n * elem);
+    if (st) return st;
+    CW_HIP(hipMemcpyAsync(*stage, src, n * elem, hipMemcpyHostToDevice, s));
+    const unsigned gb = (unsigned)((n + kBlock - 1) / kBlock);
+    if (elem == 1)
+        hipLaunchKernelGGL(k_unflatten<uint8_t>, dim3(gb), dim3(kBlock), 0, s,
+                           (const uint8_t*)*stage, (uint8_t*)dst, W, rows, P);
+    else if (elem == 4)
+        hipLaunchKernelGGL(k_unflatten<uint32_t>, dim3(gb), dim3(kBlock), 0, s,
+                           (const uint32_t*)*stage, (uint32_t*)dst, W, rows, P);
+    else
+        hipLaunchKernelGGL(k_unflatten<uint64_t>, dim3(gb), dim3(kBlock), 0, s,
+                           (const uint64_t*)*stage, (uint64_t*)dst, W, rows, P);
+    CW_HIP(hipGetLastError());
+    CW_HIP(hipStreamSynchronize(s));
+    return CW_OK;
+}
+
+cw_status copy_out(hipStream_t s, int W, size_t rows, int P, const void* src, void* dst, int elem,
+                   void** stage, size_t* cap) {
+    const size_t n = (size_t)W * rows;
+    cw_status st = stage_ensure(stage, cap, n * elem);
+    if (st) return st;
+    const unsigned gb = (unsigned)((n + kBlock - 1) / kBlock);
+    if (elem == 1)
+        hipLaunchKernelGGL(k_flatten<uint8_t>, dim3(gb), dim3(kBlock), 0, s, (const uint8_t*)src,
+                           (uint8_t*)*stage, W, rows, P);
+    else if (elem == 4)
+        hipLaunchKernelGGL(k_flatten<uint32_t>, dim3(gb), dim3(kBlock), 0, s,
+                           (const uint32_t*)src, (uint32_t*)*stage, W, rows, P);
+    else
+        hipLaunchKernelGGL(k_flatten<uint64_t>, dim3(gb), dim3(kBlock), 0, s,
+                           (const uint64_t*)src, (uint64_t*)*stage, W, rows, P);
+    CW_HIP(hipGetLastError());
+    CW_HIP(hipMemcpyAsync(dst, *stage, n * elem, hipMemcpyDeviceToHost, s));
+    CW_HIP(hipStreamSynchronize(s));
+    return CW_OK;
 }
 
 constexpr int kInflowPer = 4;  // pitched elements per thread
@@ -552,11 +652,8 @@ cw_status finish(cw_field* f) {
 cw_status upload_mask(const cw_field* f, const uint8_t* host, uint8_t* dev) {
     std::vector<uint8_t> m(f->n);
     for (size_t i = 0; i < f->n; i++) m[i] = host[i] ? 1 : 0;
-    CW_HIP(hipMemset2DAsync(dev, f->P, 0, f->P, (size_t)f->H * f->D, f->stream));
-    CW_HIP(hipMemcpy2DAsync(dev, f->P, m.data(), f->W, f->W, (size_t)f->H * f->D,
-                            hipMemcpyHostToDevice, f->stream));
-    CW_HIP(hipStreamSynchronize(f->stream));
-    return CW_OK;
+    CW_HIP(hipMemsetAsync(dev, 0, f->npad, f->stream));
+    return upload_flat(f, dev, m.data(), 1);
 }
 
 // pde.rs:550-631 with `open` already on the device.
@@ -564,7 +661,8 @@ cw_status flood_hold(cw_field* f, const uint8_t* open, uint8_t* reach, double va
     const Geo g = geo_of(f);
     const unsigned gb = grid_of(f->npad);
     hipLaunchKernelGGL(k_box_init, dim3(1), dim3(64), 0, f->stream, f->scan);
-    hipLaunchKernelGGL(k_closed_box, dim3(gb), dim3(kBlock), 0, f->stream, g, open, f->scan);
+    hipLaunchKernelGGL(k_closed_box, dim3(red_grid(f->npad)), dim3(kBlock), 0, f->stream, g, open,
+                       f->scan);
     int box[8];
     CW_HIP(hipMemcpyAsync(box, f->scan, sizeof box, hipMemcpyDeviceToHost, f->stream));
     CW_HIP(hipStreamSynchronize(f->stream));
@@ -675,11 +773,16 @@ cw_status build_coefficients(cw_field* f) {
     const Geo g = geo_of(f);
     const unsigned gb = grid_of(f->npad);
     hipLaunchKernelGGL(k_scan_init, dim3(1), dim3(64), 0, f->stream, f->scan);
-    hipLaunchKernelGGL(k_scan, dim3(gb), dim3(kBlock), 0, f->stream, g, (const uint8_t*)f->held,
+    hipLaunchKernelGGL(k_scan, dim3(red_grid(f->npad)), dim3(kBlock), 0, f->stream, g,
+                       (const uint8_t*)f->held,
                        (const double*)f->up, (const double*)f->pr, f->source, f->scan);
-    int sc[10];
-    CW_HIP(hipMemcpyAsync(sc, f->scan, sizeof sc, hipMemcpyDeviceToHost, f->stream));
+    int raw[11];
+    CW_HIP(hipMemcpyAsync(raw, f->scan, sizeof raw, hipMemcpyDeviceToHost, f->stream));
     CW_HIP(hipStreamSynchronize(f->stream));
+    // scan[10] holds the flag bits: 1 general, 2 folded sink, 4 general sink, 8 any free
+    int sc[10];
+    for (int a = 0; a < 10; a++) sc[a] = raw[a];
+    for (int b = 0; b < 4; b++) sc[b] = (raw[10] >> b) & 1;
     const bool general = f->source != 0.0 || sc[0];
     if (general && !f->k2) {
         cw_status st = alloc_fill((void**)&f->k2, f->npad * sizeof(double), f->stream, 0);
@@ -698,19 +801,22 @@ cw_status build_coefficients(cw_field* f) {
     return CW_OK;
 }
 
+cw_status upload_flat(const cw_field* f, void* dst, const void* src, int elem) {
+    return copy_in(f->stream, (int)f->W, (size_t)f->H * f->D, f->P, dst, src, elem, &f->stage,
+                   &f->stage_bytes);
+}
+
+cw_status download_flat(const cw_field* f, const void* src, void* dst, int elem) {
+    return copy_out(f->stream, (int)f->W, (size_t)f->H * f->D, f->P, src, dst, elem, &f->stage,
+                    &f->stage_bytes);
+}
+
 cw_status download_held(const cw_field* f, uint8_t* out) {
-    CW_HIP(hipMemcpy2DAsync(out, f->W, f->held, f->P, f->W, (size_t)f->H * f->D,
-                            hipMemcpyDeviceToHost, f->stream));
-    CW_HIP(hipStreamSynchronize(f->stream));
-    return CW_OK;
+    return download_flat(f, f->held, out, 1);
 }
 
 cw_status download_f64(const cw_field* f, const double* dev, double* out) {
-    CW_HIP(hipMemcpy2DAsync(out, f->W * sizeof(double), dev, f->P * sizeof(double),
-                            f->W * sizeof(double), (size_t)f->H * f->D, hipMemcpyDeviceToHost,
-                            f->stream));
-    CW_HIP(hipStreamSynchronize(f->stream));
-    return CW_OK;
+    return download_flat(f, dev, out, 8);
 }
 
 }  // namespace cwi
@@ -747,6 +853,7 @@ cw_status cw_lattice_destroy(cw_lattice* l) {
     (void)hipFree(l->ids); (void)hipFree(l->flags); (void)hipFree(l->table);
     for (int q = 0; q < 2; q++) { (void)hipFree(l->keys[q]); (void)hipFree(l->vals[q]); }
     (void)hipFree(l->first); (void)hipFree(l->last); (void)hipFree(l->sort_tmp);
+    (void)hipFree(l->stage);
     if (l->stream) (void)hipStreamDestroy(l->stream);
     delete l;
     return CW_OK;
@@ -754,10 +861,9 @@ cw_status cw_lattice_destroy(cw_lattice* l) {
 
 cw_status cw_lattice_upload(cw_lattice* l, const uint32_t* ids) {
     if (!l || !ids) return fail(CW_ERR_ARG, "null argument");
-    CW_HIP(hipMemcpy2DAsync(l->ids, l->P * sizeof(uint32_t), ids, l->W * sizeof(uint32_t),
-                            l->W * sizeof(uint32_t), (size_t)l->H * l->D, hipMemcpyHostToDevice,
-                            l->stream));
-    CW_HIP(hipStreamSynchronize(l->stream));
+    cw_status st = copy_in(l->stream, (int)l->W, (size_t)l->H * l->D, l->P, l->ids, ids, 4,
+                           &l->stage, &l->stage_bytes);
+    if (st) return st;
     l->csr_valid = false;
     return CW_OK;
 }
@@ -856,7 +962,7 @@ cw_status cw_field_free_count(cw_field* f, uint64_t* out) {
     if (!f || !out) return fail(CW_ERR_ARG, "null argument");
     unsigned long long* dev = (unsigned long long*)(f->scan + 32);
     CW_HIP(hipMemsetAsync(dev, 0, sizeof *dev, f->stream));
-    hipLaunchKernelGGL(k_count_free, dim3(grid_of(f->npad)), dim3(kBlock), 0, f->stream,
+    hipLaunchKernelGGL(k_count_free, dim3(red_grid(f->npad)), dim3(kBlock), 0, f->stream,
                        geo_of(f), (const uint8_t*)f->held, dev);
     CW_HIP(hipGetLastError());
     unsigned long long v = 0;

@@ -53,6 +53,8 @@ struct cw_field {
     float* r32 = nullptr;
     float* lb = nullptr;
     int* scan = nullptr;       // small device scratch of the coefficient scan
+    mutable void* stage = nullptr;  // flat device staging of host copies
+    mutable size_t stage_bytes = 0;
     bool has_span = false;
     int span[6] = {0, 0, 0, 0, 0, 0};
     bool has_sink = false;
@@ -67,5 +69,10 @@ cw_status build_coefficients(cw_field* f);
 // Flat (W*H*D) host copies of the device state.
 cw_status download_held(const cw_field* f, uint8_t* out);
 cw_status download_f64(const cw_field* f, const double* dev, double* out);
+// Host flat (W*H*D, x fastest) <-> device pitched copies of elem-byte
+// elements (1, 4 or 8): one contiguous PCIe copy through a flat device
+// staging buffer and a pack/unpack kernel, instead of one copy per row.
+cw_status upload_flat(const cw_field* f, void* dst, const void* src, int elem);
+cw_status download_flat(const cw_field* f, const void* src, void* dst, int elem);
 
 }  // namespace cwi

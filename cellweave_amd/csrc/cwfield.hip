@@ -1064,10 +1064,7 @@ cw_status field_adjoint(cw_field* f, const double* target, const uint32_t* grid,
         for (uint32_t k = 0; k < ncells; k++) dl_du[k] = 0.0;
     // forward state on the host: loss, the held voxels' own term, the floor
     std::vector<double> c(f->n);
-    CW_HIP(hipMemcpy2DAsync(c.data(), f->W * sizeof(double), f->c[f->cur], f->P * sizeof(double),
-                            f->W * sizeof(double), (size_t)f->H * f->D, hipMemcpyDeviceToHost,
-                            f->stream));
-    CW_HIP(hipStreamSynchronize(f->stream));
+    if ((st = download_f64(f, f->c[f->cur], c.data()))) return st;
     std::vector<uint8_t> held(f->n);
     std::vector<double> uptake;
     if ((st = download_held(f, held.data()))) return st;
@@ -1111,9 +1108,7 @@ cw_status field_adjoint(cw_field* f, const double* target, const uint32_t* grid,
         hipMemsetAsync(lam[0], 0, bytes, f->stream) != hipSuccess ||
         hipMemsetAsync(lam[1], 0, bytes, f->stream) != hipSuccess ||
         hipMemsetAsync(tgt, 0, bytes, f->stream) != hipSuccess ||
-        hipMemcpy2DAsync(tgt, f->P * sizeof(double), target, f->W * sizeof(double),
-                         f->W * sizeof(double), (size_t)f->H * f->D, hipMemcpyHostToDevice,
-                         f->stream) != hipSuccess)
+        upload_flat(f, tgt, target, 8) != CW_OK)
         return bail(fail(CW_ERR_DEVICE, "adjoint target upload failed"));
     const bool three_d = f->D > 1;
     const double omega = pick_omega(o.omega, f);
@@ -1322,12 +1317,10 @@ cw_status cw_field_create(uint32_t w, uint32_t h, uint32_t d, double diff, doubl
     if ((st = alloc_zero((void**)&f->scan, 64 * sizeof(int), f->stream, 0))) return bail(st);
     // nothing held inside the box; the padding stays held
     CW_HIP(hipMemset2DAsync(f->held, f->P, 0, f->W, (size_t)f->H * f->D, f->stream));
-    std::vector<double> init(f->n, initial);
-    if (hipMemcpy2DAsync(f->c[0], f->P * sizeof(double), init.data(), w * sizeof(double),
-                         w * sizeof(double), (size_t)h * d, hipMemcpyHostToDevice,
-                         f->stream) != hipSuccess ||
-        hipStreamSynchronize(f->stream) != hipSuccess)
-        return bail(fail(CW_ERR_DEVICE, "initial upload failed"));
+    {
+        std::vector<double> init(f->n, initial);
+        if ((st = upload_flat(f, f->c[0], init.data(), 8))) return bail(st);
+    }
     f->coef_dirty = true;
     if ((st = build_coefficients(f))) return bail(st);
     *out = f;
@@ -1338,7 +1331,7 @@ cw_status cw_field_destroy(cw_field* f) {
     if (!f) return CW_OK;
     if (f->stream) hipStreamSynchronize(f->stream);
     hipFree(f->c[0]); hipFree(f->c[1]); hipFree(f->k); hipFree(f->k2);
-    hipFree(f->up); hipFree(f->pr); hipFree(f->held); hipFree(f->scan);
+    hipFree(f->up); hipFree(f->pr); hipFree(f->held); hipFree(f->scan); hipFree(f->stage);
     hipFree(f->e[0]); hipFree(f->e[1]); hipFree(f->r32); hipFree(f->lb);
     if (f->stream) hipStreamDestroy(f->stream);
     delete f;
@@ -1355,14 +1348,11 @@ cw_status cw_field_set_source(cw_field* f, double source) {
 cw_status cw_field_upload(cw_field* f, const double* values, const uint8_t* held,
                           const double* uptake, const double* production) {
     if (!f) return fail(CW_ERR_ARG, "null field");
-    const size_t pitch8 = (size_t)f->P * sizeof(double), row8 = f->W * sizeof(double);
-    const size_t rows = (size_t)f->H * f->D;
+    cw_status st;
     if (held) {
         std::vector<uint8_t> h(f->n);
         for (size_t i = 0; i < f->n; i++) h[i] = held[i] ? 1 : 0;
-        CW_HIP(hipMemcpy2DAsync(f->held, f->P, h.data(), f->W, f->W, rows, hipMemcpyHostToDevice,
-                                f->stream));
-        CW_HIP(hipStreamSynchronize(f->stream));
+        if ((st = upload_flat(f, f->held, h.data(), 1))) return st;
         f->coef_dirty = true;
     }
     // The setters clamp at zero (pde.rs:146-182); NaN clamps to zero as
@@ -1373,28 +1363,17 @@ cw_status cw_field_upload(cw_field* f, const double* values, const uint8_t* held
         if (!src) continue;
         tmp.resize(f->n);
         for (size_t i = 0; i < f->n; i++) tmp[i] = src[i] > 0.0 ? src[i] : 0.0;
-        CW_HIP(hipMemcpy2DAsync(which ? f->pr : f->up, pitch8, tmp.data(), row8, row8, rows,
-                                hipMemcpyHostToDevice, f->stream));
-        CW_HIP(hipStreamSynchronize(f->stream));
+        if ((st = upload_flat(f, which ? f->pr : f->up, tmp.data(), 8))) return st;
         f->coef_dirty = true;
     }
-    if (values) {
-        CW_HIP(hipMemcpy2DAsync(f->c[f->cur], f->P * sizeof(double), values,
-                                f->W * sizeof(double), f->W * sizeof(double),
-                                (size_t)f->H * f->D, hipMemcpyHostToDevice, f->stream));
-        CW_HIP(hipStreamSynchronize(f->stream));
-    }
+    if (values && (st = upload_flat(f, f->c[f->cur], values, 8))) return st;
     if (f->coef_dirty) return build_coefficients(f);
     return CW_OK;
 }
 
 cw_status cw_field_download(const cw_field* f, double* values) {
     if (!f || !values) return fail(CW_ERR_ARG, "null argument");
-    CW_HIP(hipMemcpy2DAsync(values, f->W * sizeof(double), f->c[f->cur], f->P * sizeof(double),
-                            f->W * sizeof(double), (size_t)f->H * f->D, hipMemcpyDeviceToHost,
-                            f->stream));
-    CW_HIP(hipStreamSynchronize(f->stream));
-    return CW_OK;
+    return download_flat(f, f->c[f->cur], values, 8);
 }
 
 cw_status cw_field_relax(cw_field* f, double tolerance, uint64_t max_sweeps, uint64_t* sweeps) {

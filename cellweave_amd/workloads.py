@@ -317,4 +317,66 @@ def config5(n=192, ncells=25000, radius=82.0, fd_cells=8, h=1e-6, adj_tol=1e-11)
             "units": "dL/dD per px^2/s (the solve's D is um^2/s divided by dx^2 = 100 um^2)"}
 
 
-WORKLOADS = {"config1": config1, "config3": config3, "config4": config4, "config5": config5}
+def _timed(fn, reps=10):
+    fn()  # warm-up (allocations, cached cell lists)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def coupling(reps=10):
+    """The steps either side of the solve, on the device (coupling.rs:26-182,
+    pde.rs:415-728), at 1024^2 (three fields' lattice) and 256^3: wall time
+    per call (each call is synchronous) and HBM GB/s from each kernel's
+    algorithmic bytes per voxel (stated beside it)."""
+    from .field import Lattice
+
+    out = {}
+    for name, fc in (("1024^2", cases.config2(n=1024)[0]), ("256^3", cases.config3()[0])):
+        w, h, d = fc.w, fc.h, fc.d
+        n = w * h * d
+        ids = fc.grid.astype(np.uint32)
+        ncells = int(ids[ids != 0xFFFFFFFF].max()) + 1
+        consumes = np.ones(ncells, np.uint8)
+        consumes[0] = 0
+        prod = np.where(np.random.default_rng(7).random(ncells) < 0.3, 2e-4, 0.0)
+        lat = Lattice(w, h, d)
+        f = Field(w, h, d, fc.diff, 0.0, 1.0)
+        ops = {
+            # name: (call, algorithmic bytes per voxel)
+            "lattice_upload": (lambda: lat.upload(ids), 4),
+            "apply_uptake": (lambda: f.apply_uptake(lat, consumes, 1e-3), 4 + 8),
+            "apply_production": (lambda: f.apply_production(lat, prod), 4 + 8),
+            "release_all": (lambda: f.release_all(), 1),
+            "hold_medium_bath": (lambda: f.hold_medium_bath(lat, 1.0), None),
+            "free_count": (lambda: f.free_count(), 1),
+            "inflow": (lambda: f.inflow(), 8 + 1),
+            "step": (lambda: f.step(0.2 * 0.25 / max(fc.diff, 1e-30)), 8 + 1 + 8 + 8 + 8),
+            "mean_per_cell": (lambda: f.mean_per_cell(lat, ncells), 4 + 8),
+        }
+        res = {}
+        for op, (fn, bpv) in ops.items():
+            t = _timed(fn, reps)
+            r = {"ms": 1e3 * t}
+            if bpv:
+                r["bytes_per_voxel"] = bpv
+                r["GBs"] = n * bpv / t / 1e9
+                r["frac_hbm"] = r["GBs"] / HBM_PEAK_GBS
+            res[op] = r
+        # the cell lists are rebuilt after every lattice upload: time that too
+        def fresh_mean():
+            lat.upload(ids)
+            f.mean_per_cell(lat, ncells)
+        res["mean_per_cell_after_upload"] = {"ms": 1e3 * _timed(fresh_mean, reps)}
+        out[name] = {"voxels": n, "cells": ncells, "ops": res}
+    return {"workload": "device-side coupling, holds, step and per-cell means (synchronous calls)",
+            "results": out,
+            "notes": "lattice_upload includes the PCIe copy of the ids; hold_medium_bath is a "
+                     "closed-box scan, a tiled flood to a fixed point and a finish pass"}
+
+
+WORKLOADS = {"config1": config1, "config3": config3, "config4": config4, "config5": config5,
+             "coupling": coupling}
