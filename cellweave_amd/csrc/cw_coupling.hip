@@ -96,8 +96,9 @@ __global__ void k_scan_init(int* scan) {
 
 constexpr int kRedBlocks = 2048;  // grid of the reductions: grid-stride, one atomic per block
 
-unsigned red_grid(size_t npad) {
-    return (unsigned)std::min<size_t>(kRedBlocks, (npad + kBlock - 1) / kBlock);
+// Reductions walk whole rows: block b takes rows b, b + grid, ...
+unsigned red_grid(const cw_field* f) {
+    return (unsigned)std::min<size_t>(kRedBlocks, (size_t)f->H * f->D);
 }
 
 // Block-wide combine of (flags, box) and one set of atomics per block.
@@ -136,11 +137,12 @@ __global__ __launch_bounds__(kBlock) void k_scan(Geo g, const uint8_t* held, con
                                                  const double* pr, double source, int* scan) {
     int flags = 0;
     int mnx = INT_MAX, mxx = -1, mny = INT_MAX, mxy = -1, mnz = INT_MAX, mxz = -1;
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < g.npad;
-         i += (size_t)gridDim.x * kBlock) {
-        int x, y, z;
-        coords(g, i, x, y, z);
-        if (x >= g.W || held[i]) continue;
+    const size_t rows = (size_t)g.H * g.D;
+    for (size_t row = blockIdx.x; row < rows; row += gridDim.x)
+    for (int x = threadIdx.x; x < g.W; x += kBlock) {
+        const size_t i = row * (size_t)g.P + x;
+        if (held[i]) continue;
+        const int y = (int)(row % (size_t)g.H), z = (int)(row / (size_t)g.H);
         const double p = pr[i], u = up[i];
         flags |= (p != 0.0 && u != 0.0) | (((p != 0.0 ? p : -u) < 0.0) << 1) |
                  ((((source + p) - u) < 0.0) << 2) | 8;
@@ -273,11 +275,12 @@ __global__ void k_box_init(int* scan) {
 __global__ __launch_bounds__(kBlock) void k_closed_box(Geo g, const uint8_t* open, int* scan) {
     int flags = 0;
     int mnx = INT_MAX, mxx = -1, mny = INT_MAX, mxy = -1, mnz = INT_MAX, mxz = -1;
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < g.npad;
-         i += (size_t)gridDim.x * kBlock) {
-        int x, y, z;
-        coords(g, i, x, y, z);
-        if (x >= g.W || open[i]) continue;
+    const size_t rows = (size_t)g.H * g.D;
+    for (size_t row = blockIdx.x; row < rows; row += gridDim.x)
+    for (int x = threadIdx.x; x < g.W; x += kBlock) {
+        const size_t i = row * (size_t)g.P + x;
+        if (open[i]) continue;
+        const int y = (int)(row % (size_t)g.H), z = (int)(row / (size_t)g.H);
         flags = 1;
         mnx = min(mnx, x); mxx = max(mxx, x);
         mny = min(mny, y); mxy = max(mxy, y);
@@ -393,12 +396,9 @@ __global__ __launch_bounds__(kBlock) void k_count_free(Geo g, const uint8_t* hel
                                                        unsigned long long* out) {
     __shared__ unsigned long long sh[kBlock / 64];
     unsigned long long v = 0;
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < g.npad;
-         i += (size_t)gridDim.x * kBlock) {
-        int x, y, z;
-        coords(g, i, x, y, z);
-        v += (x < g.W && !held[i]) ? 1 : 0;
-    }
+    const size_t rows = (size_t)g.H * g.D;
+    for (size_t row = blockIdx.x; row < rows; row += gridDim.x)
+        for (int x = threadIdx.x; x < g.W; x += kBlock) v += held[row * (size_t)g.P + x] ? 0 : 1;
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -661,7 +661,7 @@ cw_status flood_hold(cw_field* f, const uint8_t* open, uint8_t* reach, double va
     const Geo g = geo_of(f);
     const unsigned gb = grid_of(f->npad);
     hipLaunchKernelGGL(k_box_init, dim3(1), dim3(64), 0, f->stream, f->scan);
-    hipLaunchKernelGGL(k_closed_box, dim3(red_grid(f->npad)), dim3(kBlock), 0, f->stream, g, open,
+    hipLaunchKernelGGL(k_closed_box, dim3(red_grid(f)), dim3(kBlock), 0, f->stream, g, open,
                        f->scan);
     int box[8];
     CW_HIP(hipMemcpyAsync(box, f->scan, sizeof box, hipMemcpyDeviceToHost, f->stream));
@@ -773,7 +773,7 @@ cw_status build_coefficients(cw_field* f) {
     const Geo g = geo_of(f);
     const unsigned gb = grid_of(f->npad);
     hipLaunchKernelGGL(k_scan_init, dim3(1), dim3(64), 0, f->stream, f->scan);
-    hipLaunchKernelGGL(k_scan, dim3(red_grid(f->npad)), dim3(kBlock), 0, f->stream, g,
+    hipLaunchKernelGGL(k_scan, dim3(red_grid(f)), dim3(kBlock), 0, f->stream, g,
                        (const uint8_t*)f->held,
                        (const double*)f->up, (const double*)f->pr, f->source, f->scan);
     int raw[11];
@@ -962,7 +962,7 @@ cw_status cw_field_free_count(cw_field* f, uint64_t* out) {
     if (!f || !out) return fail(CW_ERR_ARG, "null argument");
     unsigned long long* dev = (unsigned long long*)(f->scan + 32);
     CW_HIP(hipMemsetAsync(dev, 0, sizeof *dev, f->stream));
-    hipLaunchKernelGGL(k_count_free, dim3(red_grid(f->npad)), dim3(kBlock), 0, f->stream,
+    hipLaunchKernelGGL(k_count_free, dim3(red_grid(f)), dim3(kBlock), 0, f->stream,
                        geo_of(f), (const uint8_t*)f->held, dev);
     CW_HIP(hipGetLastError());
     unsigned long long v = 0;

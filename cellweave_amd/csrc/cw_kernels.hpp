@@ -1484,9 +1484,97 @@ __global__ __launch_bounds__(256) void correct64(PassArgs a) {
 // launch of a chunk (as the next sweep launch would); DECIDE_RESID turns the
 // resid64 partials into r2, b2 and stops converged problems; DECIDE_REPLAY_END
 // closes replayed problems.
-__global__ __launch_bounds__(256) void decide(PassArgs a_in) {
+// Verdict of one running problem from its residual sums (DECIDE_RESID) or
+// from launch L-1's slots (DECIDE_PASS); one thread calls it.
+__device__ inline void decide_one(const PassArgs& a, Ctl& c, bool have_r2, double r2, double b2) {
+    if (a.mode == DECIDE_RESID) {
+        c.r2 = r2;
+        c.b2 = b2;
+        if (!(r2 == r2) || r2 > 1.7e308) {
+            c.nonfinite = 1;
+            c.status = ST_MAXED;
+            c.sweeps = a.sweeps_before;
+        } else if (a.crit == CRIT_RESID && r2 <= a.tol * a.tol * (b2 > 0.0 ? b2 : 1.0)) {
+            c.status = ST_DONE;
+            c.sweeps = a.sweeps_before;
+            c.parity = a.pass & 1;
+        }
+        return;
+    }
+    const Verdict v = judge(a, c, have_r2, r2);
+    if (v.status != ST_RUNNING) {
+        if (have_r2) c.r2 = r2;
+        record(c, v);
+    }
+}
+
+// Partials a problem's verdict sums, or cnt = 0 when it needs none.
+__device__ inline void decide_partials(const PassArgs& a, const Prob& pr, bool* have_r2,
+                                       const double** part, int* base, int* cnt) {
+    *part = a.partials;
+    *base = 0;
+    *cnt = 0;
+    *have_r2 = false;
+    if (a.mode == DECIDE_RESID) {
+        *base = pr.el_base;
+        *cnt = pr.el_count;
+        *have_r2 = true;
+    } else if (a.prev_check && a.crit == CRIT_RESID && a.pass > 0) {
+        *part = a.partials + ((a.pass - 1) & 1) * a.part_stride;
+        *base = pr.sw_base[a.prev_tl];
+        *cnt = pr.sw_count[a.prev_tl];
+        *have_r2 = true;
+    }
+}
+
+constexpr int kDecideThreads = 1024;
+constexpr int kDecideShared = 4;  // up to this many problems: the whole block sums each one
+
+// One workgroup of kDecideThreads: records verdicts for the host.
+// DECIDE_PASS judges the last launch of a chunk (as the next sweep launch
+// would); DECIDE_RESID turns the resid64 partials into r2, b2 and stops
+// converged problems; DECIDE_REPLAY_END closes replayed problems. A few
+// problems are summed by the whole block in turn (a large field has tens of
+// thousands of partials); many are summed one per wave. Either way each sum
+// has a fixed order.
+__global__ __launch_bounds__(kDecideThreads) void decide(PassArgs a_in) {
     const PassArgs a = resolve(a_in);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if (a.mode != DECIDE_REPLAY_END && a.nprob <= kDecideShared) {
+        __shared__ double sr[kDecideThreads / 64], sb[kDecideThreads / 64];
+        for (int p = 0; p < a.nprob; p++) {
+            const Prob& pr = a.probs[p];
+            Ctl& c = a.ctl[p];
+            const bool running = c.status == ST_RUNNING;
+            bool have_r2;
+            const double* part;
+            int base, cnt;
+            decide_partials(a, pr, &have_r2, &part, &base, &cnt);
+            double r = 0.0, b = 0.0;
+            if (running)
+                for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+                    r += part[2 * (base + t)];
+                    b += part[2 * (base + t) + 1];
+                }
+            r = wave_sum(r);
+            b = wave_sum(b);
+            if (lane == 0) {
+                sr[wv] = r;
+                sb[wv] = b;
+            }
+            __syncthreads();
+            if (threadIdx.x == 0 && running) {
+                double r2 = 0.0, b2 = 0.0;
+                for (int w = 0; w < nw; w++) {
+                    r2 += sr[w];
+                    b2 += sb[w];
+                }
+                decide_one(a, c, have_r2, r2, b2);
+            }
+            __syncthreads();
+        }
+        return;
+    }
     for (int p = wv; p < a.nprob; p += nw) {
         const Prob& pr = a.probs[p];
         Ctl& c = a.ctl[p];
@@ -1500,37 +1588,13 @@ __global__ __launch_bounds__(256) void decide(PassArgs a_in) {
             continue;
         }
         if (c.status != ST_RUNNING) continue;
-        if (a.mode == DECIDE_RESID) {
-            double r2, b2;
-            problem_sums(a.partials, pr.el_base, pr.el_count, &r2, &b2);
-            if (lane == 0) {
-                c.r2 = r2;
-                c.b2 = b2;
-                if (!(r2 == r2) || r2 > 1.7e308) {
-                    c.nonfinite = 1;
-                    c.status = ST_MAXED;
-                    c.sweeps = a.sweeps_before;
-                } else if (a.crit == CRIT_RESID &&
-                           r2 <= a.tol * a.tol * (b2 > 0.0 ? b2 : 1.0)) {
-                    c.status = ST_DONE;
-                    c.sweeps = a.sweeps_before;
-                    c.parity = a.pass & 1;
-                }
-            }
-            continue;
-        }
-        const bool have_r2 = a.prev_check && a.crit == CRIT_RESID && a.pass > 0;
+        bool have_r2;
+        const double* part;
+        int base, cnt;
+        decide_partials(a, pr, &have_r2, &part, &base, &cnt);
         double r2 = 0.0, b2 = 0.0;
-        if (have_r2)
-            problem_sums(a.partials + ((a.pass - 1) & 1) * a.part_stride, pr.sw_base[a.prev_tl],
-                         pr.sw_count[a.prev_tl], &r2, &b2);
-        if (lane == 0) {
-            const Verdict v = judge(a, c, have_r2, r2);
-            if (v.status != ST_RUNNING) {
-                if (have_r2) c.r2 = r2;
-                record(c, v);
-            }
-        }
+        if (cnt) problem_sums(part, base, cnt, &r2, &b2);
+        if (lane == 0) decide_one(a, c, have_r2, r2, b2);
     }
 }
 

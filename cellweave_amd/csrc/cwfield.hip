@@ -378,7 +378,7 @@ cw_status launch_half3d(Plan& pl, PassArgs a, int colour) {
 
 cw_status launch_decide(Plan& pl, int mode, int crit, double tol, int max_sweeps) {
     PassArgs a = make_args(pl, TL_MAIN, mode, crit, tol, max_sweeps);
-    hipLaunchKernelGGL(decide, dim3(1), dim3(256), 0, pl.stream, a);
+    hipLaunchKernelGGL(decide, dim3(1), dim3(kDecideThreads), 0, pl.stream, a);
     CW_HIP(hipGetLastError());
     return CW_OK;
 }
@@ -484,7 +484,7 @@ cw_status capture_chunk(Plan& pl, int K, int crit, int ce, double tol, int maxs)
         a.prev_tl = TL_MAIN;
         a.prev_check = pl.three_d ? 0 : pattern_check(K - 1);
         a.run = pl.run;
-        hipLaunchKernelGGL(decide, dim3(1), dim3(256), 0, pl.stream, a);
+        hipLaunchKernelGGL(decide, dim3(1), dim3(kDecideThreads), 0, pl.stream, a);
         hipLaunchKernelGGL(advance_run, dim3(1), dim3(64), 0, pl.stream, pl.run, K, K * S);
     }
     hipGraph_t g = nullptr;
