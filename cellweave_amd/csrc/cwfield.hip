@@ -672,7 +672,10 @@ cw_status check_opts(const cw_relax_opts* o, Opts* out) {
     out->precision = o->precision;
     out->s = o->sweeps_per_pass;
     out->check_every = o->check_every > 0 ? o->check_every : 0;
-    out->inner_tol = o->inner_tolerance > 0.0 ? o->inner_tolerance : 1e-3;
+    // The FP32 smoother at omega near 2 stalls after a 1e-3..1e-4 reduction of
+    // the defect; asking each segment for 2e-2 keeps every segment at the full
+    // SOR rate (fewest total sweeps on planes, volumes and campaigns alike).
+    out->inner_tol = o->inner_tolerance > 0.0 ? o->inner_tolerance : 2e-2;
     out->max_outer = o->max_outer > 0 ? o->max_outer : 1000;
     out->shape = o->tile_shape;
     out->use_graph = o->use_graph != 0;

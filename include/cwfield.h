@@ -69,7 +69,7 @@ typedef struct cw_relax_opts {
     int32_t sweeps_per_pass;  /* temporal blocking depth s (full sweeps per
                                  HBM pass); 0 = library choice */
     int32_t check_every;      /* residual test every k passes; 0 = library choice */
-    double inner_tolerance;   /* FP32_REFINE: inner reduction per outer step; 0 = default */
+    double inner_tolerance;   /* FP32_REFINE: reduction asked of one segment; 0 = 2e-2 */
     uint64_t max_outer;       /* FP32_REFINE: outer iteration cap; 0 = default */
     int32_t tile_shape;       /* sweep kernel shape index (-1 = from
                                  sweeps_per_pass); cw_autotune ranks them */
