@@ -912,19 +912,28 @@ __global__ __launch_bounds__(NW * 64) void relax2d_reg(PassArgs a_in) {
     }
 
     RegCells<T, NR> R;
+    // Issue every row's loads before using any: a row outside the field
+    // reads element 0 (always in bounds) and is masked below, so no load
+    // waits behind another row's branch.
+    T2 lc[NR], lu[NR], lv[NR];
+    const T* xin = (const T*)pr.x[a.pass & 1];
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int gy = gyo + r0 + i;
+        const bool in = colin && gy >= 0 && gy < Hh;
+        const size_t off = in ? (size_t)gy * P + gx : 0;
+        lc[i] = *(const T2*)(kk + off);
+        if (has_k2) lu[i] = *(const T2*)(k2p + off);
+        if (!replay) lv[i] = *(const T2*)(xin + off);
+    }
 #pragma unroll
     for (int i = 0; i < NR; i++) {
         const int gy = gyo + r0 + i;
         T ve = T(0), vo = T(0), ce = nan_of<T>(), co = nan_of<T>(), ue = T(0), uo = T(0);
         if (colin && gy >= 0 && gy < Hh) {
-            const size_t off = (size_t)gy * P + gx;
-            const T2 c = *(const T2*)(kk + off);
-            ce = c.x; co = c.y;
-            if (has_k2) { const T2 u = *(const T2*)(k2p + off); ue = u.x; uo = u.y; }
-            if (!replay) {
-                const T2 v = *(const T2*)((const T*)pr.x[a.pass & 1] + off);
-                ve = v.x; vo = v.y;
-            }
+            ce = lc[i].x; co = lc[i].y;
+            if (has_k2) { ue = lu[i].x; uo = lu[i].y; }
+            if (!replay) { ve = lv[i].x; vo = lv[i].y; }
         }
         // colour of the even column on row i is base ^ (i & 1)
         const bool sw = (base ^ (i & 1)) != 0;
