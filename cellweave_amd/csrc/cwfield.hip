@@ -99,11 +99,31 @@ constexpr int kNumSmoothShapes = (int)(sizeof(kSmoothShapes) / sizeof(kSmoothSha
 // launches per sweep); kind 1 = z-streaming relax3d_stream with NW waves of
 // NR rows and zc planes per chunk (one launch per sweep, ping-pong).
 struct Shape3 { int kind, nw, nr, zc; };
-constexpr Shape3 k3DShapes[] = {{0, 4, 1, 1}, {1, 8, 4, 16}, {1, 4, 8, 16}, {1, 8, 4, 32}, {1, 8, 4, 8}};
+constexpr Shape3 k3DShapes[] = {{0, 4, 1, 1},  {1, 8, 4, 16}, {1, 4, 8, 16}, {1, 8, 4, 32},
+                                 {1, 8, 4, 8},  {1, 8, 4, 24}, {1, 4, 8, 32}, {1, 8, 4, 48},
+                                 {1, 4, 8, 24}};
 constexpr int kNum3DShapes = (int)(sizeof(k3DShapes) / sizeof(k3DShapes[0]));
 constexpr int kDefault3DShape = 1;
 
 int default_plain_shape(int s) { return s <= 1 ? 2 : 3; }
+
+// FP32 volumes: the z-streaming smoother is latency-bound, so workgroups
+// matter more than the halo planes a shorter chunk re-reads. 16-plane chunks
+// unless they leave fewer than 384 workgroups for the batch, then 8-plane.
+constexpr int kVolumeChunk16 = 1, kVolumeChunk8 = 4;
+int default_smooth3d_shape(cw_field* const* F, uint32_t nf) {
+    long tiles = 0;
+    const Shape3 sh = k3DShapes[kVolumeChunk16];
+    const int ty = sh.nw * sh.nr - 4;
+    for (uint32_t i = 0; i < nf; i++) {
+        const cw_field* f = F[i];
+        if (!f->has_span) continue;
+        const int sx = f->span[1] - f->span[0] + 1, sy = f->span[3] - f->span[2] + 1;
+        const int sz = f->span[5] - f->span[4] + 1;
+        tiles += (long)((sx + 123) / 124) * ((sy + ty - 1) / ty) * ((sz + sh.zc - 1) / sh.zc);
+    }
+    return tiles < 384 ? kVolumeChunk8 : kVolumeChunk16;
+}
 int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 : 15; }
 
 }  // namespace
@@ -654,11 +674,13 @@ cw_status check_opts(const cw_relax_opts* o, Opts* out) {
         (o->sweeps_per_pass & (o->sweeps_per_pass - 1)) != 0)
         return fail(CW_ERR_ARG, "sweeps_per_pass must be 0, 1, 2, 4 or 8");
     {
+        // planes index the 2D table of the precision, volumes the 3D table;
+        // the fields' dimensionality is checked where the plan is built
         const bool plain = o->precision == CW_PREC_FP64;
-        const int nshape = plain ? kNumPlainShapes : kNumSmoothShapes;
-        if (o->tile_shape >= nshape)
+        const int n2 = plain ? kNumPlainShapes : kNumSmoothShapes;
+        if (o->tile_shape >= std::max(n2, kNum3DShapes))
             return fail(CW_ERR_ARG, "unknown tile shape %d", o->tile_shape);
-        const int ts = o->tile_shape < 0 ? 0
+        const int ts = (o->tile_shape < 0 || o->tile_shape >= n2) ? 0
                        : plain ? kPlainShapes[o->tile_shape].s
                                : kSmoothShapes[o->tile_shape].s;
         if (o->tile_shape >= 0 && o->sweeps_per_pass && ts != o->sweeps_per_pass)
@@ -703,8 +725,11 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     pl.three_d = three_d;
     pl.s_main = s;
     pl.shape = o.shape >= 0 ? o.shape : default_plain_shape(s);
-    if (pl.shape >= kNumPlainShapes) return fail(CW_ERR_ARG, "unknown tile shape %d", pl.shape);
-    pl.s_main = kPlainShapes[pl.shape].s;
+    if (!three_d && pl.shape >= kNumPlainShapes)
+        return fail(CW_ERR_ARG, "unknown tile shape %d", pl.shape);
+    if (three_d && o.shape >= kNum3DShapes)
+        return fail(CW_ERR_ARG, "unknown volume tile shape %d", o.shape);
+    pl.s_main = three_d ? 1 : kPlainShapes[pl.shape].s;
     pl.use_graph = o.use_graph;
     if (three_d) {
         // volumes: tile_shape picks a 3D shape, one sweep per launch
@@ -868,7 +893,10 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         if ((F[i]->D > 1) != three_d)
             return fail(CW_ERR_ARG, "a batch mixes planes and volumes");
     int shape = o.shape >= 0 ? o.shape : default_smooth_shape(o.s ? o.s : 8);
-    if (shape >= kNumSmoothShapes) return fail(CW_ERR_ARG, "unknown tile shape %d", shape);
+    if (!three_d && shape >= kNumSmoothShapes)
+        return fail(CW_ERR_ARG, "unknown tile shape %d", shape);
+    if (three_d && o.shape >= kNum3DShapes)
+        return fail(CW_ERR_ARG, "unknown volume tile shape %d", o.shape);
     if (three_d) shape = 0;
     const int s = kSmoothShapes[shape].s;
     Plan outer, inner;
@@ -880,7 +908,9 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     inner.s_main = s;
     inner.shape = shape;
     inner.use_graph = o.use_graph;
-    if (three_d) inner.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape : kDefault3DShape;
+    if (three_d)
+        inner.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape
+                                                                 : default_smooth3d_shape(F, nf);
     std::vector<int> idx;
     std::vector<double> omegas(nf);
     int longest = 2;
