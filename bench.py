@@ -49,12 +49,41 @@ def bytes_per_voxel_pass(fc, precision):
     return 16.0 if has_sink else 12.0
 
 
-def cpu_baseline(fc, budget_s=12.0):
+def copy_bandwidth(gib=1.0):
+    """Attainable HBM bandwidth: a device-to-device copy of `gib` GiB (read +
+    write bytes over the copy kernel's time, torch events on its stream)."""
+    try:
+        import torch
+
+        if not torch.cuda.is_available():
+            return None
+        n = int(gib * (1 << 30)) // 4
+        a = torch.empty(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 10
+        e0.record()
+        for _ in range(reps):
+            b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        del a, b
+        return 2.0 * n * 4 / (ms * 1e-3) / 1e9
+    except Exception:
+        return None
+
+
+def cpu_baseline(fc, budget_s=12.0, threads=None):
     """The oracle (C restatement of pde.rs, OpenMP over rows in chunks of 4)
     timed on this host: sweeps of the oxygen field until ~budget_s of work."""
     from oracle.orf import OracleField
 
-    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
+    if threads is None:
+        threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
     f = OracleField(fc.w, fc.h, fc.d, fc.diff, fc.decay, fc.initial)
     f.values[:] = fc.initial_values()
     f.copy_next_from_values()
@@ -198,6 +227,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline")
         cpu = cpu_baseline(fcs[0])
+        one = cpu_baseline(fcs[0], budget_s=4.0, threads=1)
+        cpu["single_thread"] = {"value": one["value"], "sample": one["sample"]}
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_fp32_smoother_config2.json")
@@ -229,6 +260,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": f"relax2d_reg<float,{spp_used},...> (FP32 smoother, {spp_used} sweeps per launch)",
             "alg_bytes_per_launch": bytes32, "launch_ms": ms32,
+            "attainable_copy_GBs": copy_bandwidth(),
         },
         "cpu_baseline": cpu,
         "fp64": {
