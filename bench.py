@@ -116,18 +116,19 @@ def main():
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--time-launches", type=int, default=200)
-    ap.add_argument("--workload", default="config2",
-                    help="config2 (default line) or config1/config3/config4/config5")
+    ap.add_argument("--workload", default="multifield1024",
+                    help="multifield1024 (the default line) or oxygen200, spheroid256, campaign512, "
+                         "adjoint192, coupling")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the batched and single solves shard without exchange; see DESIGN.md)",
-              file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: every GPU box "
+              "here has one GPU (a batch of fields shards across GPUs with no exchange, a single "
+              "field by slabs with a halo exchange per launch)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
 
-    if args.workload != "config2":
+    if args.workload != "multifield1024":
         from cellweave_amd import workloads
 
         print(json.dumps(workloads.WORKLOADS[args.workload]()))
@@ -144,7 +145,7 @@ def main():
         sync = lambda: None  # noqa: E731
 
     log(f"building workload: {args.n}^2 plane, three fields")
-    fcs = cases.config2(n=args.n)
+    fcs = cases.multifield_plane(n=args.n)
     fields = []
     for fc in fcs:
         g = Field(fc.w, fc.h, fc.d, fc.diff, fc.decay, fc.initial)
@@ -231,7 +232,7 @@ def main():
         cpu["single_thread"] = {"value": one["value"], "sample": one["sample"]}
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_fp32_smoother_config2.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic_fp32_smoother_multifield_1024.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
@@ -248,7 +249,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 smoothing, f64 residual/correction",
-        "data": "synthetic (seeded Voronoi lattice, config 2 of the workload list)",
+        "data": "synthetic (seeded Voronoi lattice, 1024^2 plane, ~40k cells, three fields)",
         "config": {
             "workload": f"2D multi-field {args.n}x{args.n}, ~40k cells, oxygen+signal+chemokine, "
                         "FP32+refine to relative residual 1e-8",

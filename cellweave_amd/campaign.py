@@ -113,11 +113,11 @@ def main():
     args = ap.parse_args()
     from . import cases
 
-    hist = cases.config4(count=args.count)
+    hist = cases.campaign_histories(count=args.count)
     flat = [fc for h in hist for fc in h]
     if args.tune:
         table = tune(flat)
-        table["workload"] = f"config 4 sample: {args.count} histories x 2 fields"
+        table["workload"] = f"campaign sample: {args.count} histories x 2 fields"
         with open(TABLE_PATH, "w") as fh:
             json.dump(table, fh, indent=1)
         print(json.dumps(table))

@@ -148,7 +148,7 @@ def _field(name, w, h, d, diff, decay, initial, hold, uptake, production, grid=N
                      seed=seed)
 
 
-def config1(seed: int = 1000):
+def oxygen_disc_200(seed: int = 1000):
     """200^2 plane, one oxygen field, ~450 Voronoi cells in a radius-60 disc,
     per-cell uptake U[0.5, 1.5]e-3 /s, ring held at 1. The solve uses omega 1.9."""
     rng = np.random.default_rng(seed)
@@ -190,7 +190,7 @@ def _plane_fields(n, radius, nseeds, rng, chemokine=True, seed=0):
 CHEMOKINE_STRENGTH = 1e-3
 
 
-def config2(seed: int = 2000, n: int = 1024):
+def multifield_plane(seed: int = 2000, n: int = 1024):
     """1024^2 plane, ~40k Voronoi cells in a radius-450 disc, three fields:
     oxygen (D 2000 um^2/s, uptake U[0.5,1.5]e-3), secreted signal (D 100,
     release U[0,1e-4] in 30% of cells, decay 1e-4), chemokine (D 50, 64 point
@@ -202,7 +202,7 @@ def config2(seed: int = 2000, n: int = 1024):
                          seed=seed)
 
 
-def config3(seed: int = 3000, n: int = 256, ncells: int = 60000, radius: float = 110.0,
+def spheroid(seed: int = 3000, n: int = 256, ncells: int = 60000, radius: float = 110.0,
             necrotic: float = 40.0):
     """256^3 volume, ~60k Voronoi cells in a radius-110 ball, lognormal per-cell
     uptake (mean 1e-3 /s, sigma 0.3), none within radius 40 of the centre,
@@ -221,10 +221,10 @@ def config3(seed: int = 3000, n: int = 256, ncells: int = 60000, radius: float =
                    grid, rates, seed)]
 
 
-def config4(seed: int = 4000, count: int = 512, lo: int = 128, hi: int = 768):
+def campaign_histories(seed: int = 4000, count: int = 512, lo: int = 128, hi: int = 768):
     """`count` independent plane histories, edge n = round(exp(U[ln lo, ln hi])),
     a centred disc covering U[0.4, 0.8] of the grid, Voronoi cells at the
-    density of config 2, oxygen and signal fields each."""
+    density of the 1024^2 multi-field plane, oxygen and signal fields each."""
     rng = np.random.default_rng(seed)
     density = 40000.0 / (math.pi * 450.0 * 450.0)
     out = []
@@ -238,12 +238,12 @@ def config4(seed: int = 4000, count: int = 512, lo: int = 128, hi: int = 768):
     return out
 
 
-def config5(seed: int = 5000, n: int = 192, ncells: int = 25000, radius: float = 82.0,
+def adjoint_spheroid(seed: int = 5000, n: int = 192, ncells: int = 25000, radius: float = 82.0,
             necrotic: float = 40.0):
-    """192^3 oxygen volume set up as config3 with ~25k cells. Returns the field
+    """192^3 oxygen volume set up as the 256^3 spheroid, with ~25k cells. Returns the field
     and the target's parameters: D scaled by 1.1 and each cell's uptake by
     (1 + 0.1 U[-1, 1])."""
-    (field,) = config3(seed=seed, n=n, ncells=ncells, radius=radius, necrotic=necrotic)
+    (field,) = spheroid(seed=seed, n=n, ncells=ncells, radius=radius, necrotic=necrotic)
     rng = np.random.default_rng(seed + 1)
     noise = 1.0 + 0.1 * rng.uniform(-1.0, 1.0, field.per_cell_uptake.size)
     noise[0] = 1.0

@@ -173,7 +173,7 @@ struct Verdict {
 };
 
 // Fate of a running problem after launch a.pass - 1.
-__device__ inline Verdict judge(const PassArgs& a, const Ctl& c, bool have_r2, double r2) {
+__device__ inline Verdict assess(const PassArgs& a, const Ctl& c, bool have_r2, double r2) {
     Verdict v{ST_RUNNING, a.pass & 1, a.sweeps_before, 0, 0};
     if (a.pass == 0) return v;
     const int lp = a.pass - 1;
@@ -240,7 +240,7 @@ __device__ inline bool sweep_prologue(const PassArgs& a, const Prob& pr, int p, 
     if (threadIdx.x == 0) {
         int run = 0;
         if (c.status == ST_RUNNING) {
-            const Verdict v = judge(a, c, have_r2, r2);
+            const Verdict v = assess(a, c, have_r2, r2);
             run = v.status == ST_RUNNING;
             if (!run && lt == 0) {
                 if (have_r2) c.r2 = r2;
@@ -1489,7 +1489,7 @@ __global__ __launch_bounds__(256) void correct64(PassArgs a) {
     }
 }
 
-// One workgroup: records verdicts for the host. DECIDE_PASS judges the last
+// One workgroup: records verdicts for the host. DECIDE_PASS assesses the last
 // launch of a chunk (as the next sweep launch would); DECIDE_RESID turns the
 // resid64 partials into r2, b2 and stops converged problems; DECIDE_REPLAY_END
 // closes replayed problems.
@@ -1510,7 +1510,7 @@ __device__ inline void decide_one(const PassArgs& a, Ctl& c, bool have_r2, doubl
         }
         return;
     }
-    const Verdict v = judge(a, c, have_r2, r2);
+    const Verdict v = assess(a, c, have_r2, r2);
     if (v.status != ST_RUNNING) {
         if (have_r2) c.r2 = r2;
         record(c, v);
@@ -1540,7 +1540,7 @@ constexpr int kDecideThreads = 1024;
 constexpr int kDecideShared = 4;  // up to this many problems: the whole block sums each one
 
 // One workgroup of kDecideThreads: records verdicts for the host.
-// DECIDE_PASS judges the last launch of a chunk (as the next sweep launch
+// DECIDE_PASS assesses the last launch of a chunk (as the next sweep launch
 // would); DECIDE_RESID turns the resid64 partials into r2, b2 and stops
 // converged problems; DECIDE_REPLAY_END closes replayed problems. A few
 // problems are summed by the whole block in turn (a large field has tens of

@@ -5,10 +5,11 @@ bench.py's JSON line: voxel-sweeps/s (swept voxels x sweeps / wall time), the
 FP64 solve beside the FP32+refine one, and the live per-launch timing of the
 dominant kernel for the roofline.
 
-    config1: 200^2, one oxygen field, omega 1.9 (the reference's CPU-sized case)
-    config3: 256^3 spheroid, FP32+refine against FP64
-    config4: 512 histories x 2 fields, batched per size class, graph replays
-    config5: 192^3 adjoint, gradients against central finite differences
+    oxygen200:    200^2, one oxygen field, omega 1.9 (the reference's CPU-sized case)
+    spheroid256:  256^3 spheroid, FP32+refine against FP64
+    campaign512:  512 histories x 2 fields, batched per size class, graph replays
+    adjoint192:   192^3 adjoint, gradients against central finite differences
+    coupling:     the device-side gathers, holds, step and per-cell means
 """
 from __future__ import annotations
 
@@ -78,8 +79,8 @@ def _strip(res):
     return {k: {kk: vv for kk, vv in v.items() if kk != "values"} for k, v in res.items()}
 
 
-def config1():
-    (fc,) = cases.config1()
+def oxygen200():
+    (fc,) = cases.oxygen_disc_200()
     g = _field(fc)
     res = _solve_pair([g], [fc], 1.9, lambda: g.fill(1.0))
     vs = res["fp32_refine"]["voxel_sweeps_per_s"]
@@ -98,9 +99,9 @@ def config1():
                   "errors_vs_fp64": _errors(res, [fc]), "reference_criterion": ref_rule})
 
 
-def config3(n=256):
+def spheroid256(n=256):
     t0 = time.perf_counter()
-    (fc,) = cases.config3(n=n, ncells=int(round(60000 * (n / 256) ** 3)), radius=110 * n / 256,
+    (fc,) = cases.spheroid(n=n, ncells=int(round(60000 * (n / 256) ** 3)), radius=110 * n / 256,
                           necrotic=40 * n / 256)
     build = time.perf_counter() - t0
     g = _field(fc)
@@ -135,9 +136,9 @@ def config3(n=256):
                       res["fp32_refine"]["ms_per_solve"]}})
 
 
-def config4(count=512, reps=2):
+def campaign512(count=512, reps=2):
     t0 = time.perf_counter()
-    hist = cases.config4(count=count)
+    hist = cases.campaign_histories(count=count)
     flat = [fc for h in hist for fc in h]
     build = time.perf_counter() - t0
     fields = campaign.make_fields(flat)
@@ -210,7 +211,7 @@ class SpheroidAdjoint:
 
     def __init__(self, n=192, ncells=25000, radius=82.0, necrotic=40.0):
         self.n = n
-        self.fc, self.pert = cases.config5(n=n, ncells=ncells, radius=radius, necrotic=necrotic)
+        self.fc, self.pert = cases.adjoint_spheroid(n=n, ncells=ncells, radius=radius, necrotic=necrotic)
         self.necro = cases.necrotic_mask(n, necrotic)
         self.held = self.fc.held == 1
         self.rates = self.fc.per_cell_uptake
@@ -277,7 +278,7 @@ class SpheroidAdjoint:
         return out
 
 
-def config5(n=192, ncells=25000, radius=82.0, fd_cells=32, h=1e-6, adj_tol=1e-11):
+def adjoint192(n=192, ncells=25000, radius=82.0, fd_cells=32, h=1e-6, adj_tol=1e-11):
     """Forward in FP32+refinement, adjoint in FP64, gradients against central
     differences at relative step h (correction-form perturbed solves)."""
     t0 = time.perf_counter()
@@ -345,7 +346,7 @@ def coupling(reps=10):
     from .field import Lattice
 
     out = {}
-    for name, fc in (("1024^2", cases.config2(n=1024)[0]), ("256^3", cases.config3()[0])):
+    for name, fc in (("1024^2", cases.multifield_plane(n=1024)[0]), ("256^3", cases.spheroid()[0])):
         w, h, d = fc.w, fc.h, fc.d
         n = w * h * d
         ids = fc.grid.astype(np.uint32)
@@ -388,5 +389,5 @@ def coupling(reps=10):
                      "closed-box scan, a tiled flood to a fixed point and a finish pass"}
 
 
-WORKLOADS = {"config1": config1, "config3": config3, "config4": config4, "config5": config5,
-             "coupling": coupling}
+WORKLOADS = {"oxygen200": oxygen200, "spheroid256": spheroid256, "campaign512": campaign512,
+             "adjoint192": adjoint192, "coupling": coupling}

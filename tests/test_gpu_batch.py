@@ -18,7 +18,7 @@ def say(msg):
 @pytest.mark.parametrize("crit,tol", [(CRIT_MAX_CHANGE, 1e-10), (CRIT_REL_RESIDUAL, 1e-9)])
 def test_graph_replay_equals_eager(crit, tol):
     say(f"case graph vs eager crit={crit}")
-    fcs = cases.config2(n=192) + cases.config1()
+    fcs = cases.multifield_plane(n=192) + cases.oxygen_disc_200()
     eager = [gpu_from_case(fc) for fc in fcs]
     st_e, _ = batch_relax(eager, criterion=crit, tolerance=tol, max_sweeps=200_000, omega=1.95)
     graph = [gpu_from_case(fc) for fc in fcs]
@@ -35,7 +35,7 @@ def test_graph_replay_equals_eager(crit, tol):
 
 def test_graph_refine_equals_eager():
     say("case graph refine")
-    fcs = cases.config2(n=192)
+    fcs = cases.multifield_plane(n=192)
     e = [gpu_from_case(fc) for fc in fcs]
     g = [gpu_from_case(fc) for fc in fcs]
     st_e, _ = batch_relax(e, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, omega=1.96,
@@ -50,7 +50,7 @@ def test_graph_refine_equals_eager():
 @pytest.mark.parametrize("precision", [PREC_FP64, PREC_FP32_REFINE])
 def test_small_campaign_converges_and_matches_single_solves(precision):
     say(f"case small campaign precision={precision}")
-    hist = cases.config4(count=12, lo=48, hi=160)
+    hist = cases.campaign_histories(count=12, lo=48, hi=160)
     flat = [fc for h in hist for fc in h]
     fields = campaign.make_fields(flat)
     stats, secs = campaign.solve(fields, campaign.plan(flat, precision, table={}),

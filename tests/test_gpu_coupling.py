@@ -164,8 +164,8 @@ def test_bath_fixtures_bitwise(name, shape, open_mask):
 
 
 def test_medium_bath_from_the_lattice_bitwise():
-    say("case hold_medium_bath config2 lattice 512^2")
-    (fc, *_rest) = cases.config2(n=512)
+    say("case hold_medium_bath multi-field lattice 512^2")
+    (fc, *_rest) = cases.multifield_plane(n=512)
     ids = fc.grid.astype(np.uint32)
     ref, g = pair((512, 512, 1), initial=0.5)
     lat = Lattice(512, 512, 1)
@@ -294,8 +294,8 @@ def test_device_coupling_then_relax_equals_the_oracle():
     """The loop around the solve (simulation.rs:2138-2162 declare_sources, then
     relax, then mean_per_cell) done on the device, against the oracle doing
     the same on the host: values, sweep count and per-cell means bitwise."""
-    say("case device coupling + relax, config2 oxygen 256^2")
-    oxygen, signal, _chem = cases.config2(n=256)
+    say("case device coupling + relax, multi-field oxygen 256^2")
+    oxygen, signal, _chem = cases.multifield_plane(n=256)
     ids = oxygen.grid.astype(np.uint32)
     ncells = int(ids[ids != OBSTACLE].max()) + 1
     consumes = np.ones(ncells, np.uint8)

@@ -42,8 +42,8 @@ PLAIN_SHAPES = range(7)  # FP64 plain sweep kernels (LDS tiles and register-bloc
 
 @pytest.mark.parametrize("shape", PLAIN_SHAPES)
 def test_config1_max_change_bitwise(shape):
-    say(f"case config1 omega=1.9 plain shape={shape}")
-    (fc,) = cases.config1()
+    say(f"case oxygen 200^2 omega=1.9 plain shape={shape}")
+    (fc,) = cases.oxygen_disc_200()
     ref = oracle_from_case(fc)
     g = gpu_from_case(fc)
     want = ref.relax(1e-10, 100_000, omega=1.9)
@@ -65,8 +65,8 @@ def test_general_form_every_shape(shape):
 
 
 def test_config1_reference_dropin():
-    say("case config1 reference relax (omega rule 1.95)")
-    (fc,) = cases.config1()
+    say("case oxygen 200^2 reference relax (omega rule 1.95)")
+    (fc,) = cases.oxygen_disc_200()
     ref = oracle_from_case(fc)
     g = gpu_from_case(fc)
     assert g.relax(1e-9, 100_000) == ref.relax(1e-9, 100_000)
@@ -75,8 +75,8 @@ def test_config1_reference_dropin():
 
 @pytest.mark.parametrize("shape", [0, 2, 6])
 def test_config1_rel_residual_bitwise(shape):
-    say(f"case config1 rel residual 1e-8 one sweep per pass, shape={shape}")
-    (fc,) = cases.config1()
+    say(f"case oxygen 200^2 rel residual 1e-8 one sweep per pass, shape={shape}")
+    (fc,) = cases.oxygen_disc_200()
     ref = oracle_from_case(fc)
     g = gpu_from_case(fc)
     want = ref.relax_residual(1e-8, 100_000, omega=1.9)
@@ -165,7 +165,7 @@ def test_settled_field_takes_one_sweep():
 
 def test_batch_equals_single():
     say("case batch of three fields 256^2")
-    fields = cases.config2(n=256)
+    fields = cases.multifield_plane(n=256)
     singles, batched = [], []
     for fc in fields:
         g = gpu_from_case(fc)
@@ -184,9 +184,9 @@ SMOOTH_SHAPES = range(18)  # FP32 smoother kernels
 
 
 @pytest.mark.parametrize("shape", SMOOTH_SHAPES)
-def test_fp32_refine_reaches_residual_config1(shape):
-    say(f"case config1 fp32 refine shape={shape}")
-    (fc,) = cases.config1()
+def test_fp32_refine_reaches_residual_oxygen_200(shape):
+    say(f"case oxygen 200^2 fp32 refine shape={shape}")
+    (fc,) = cases.oxygen_disc_200()
     ref = oracle_from_case(fc)
     ref.relax_residual(1e-12, 200_000, omega=1.9)
     g = gpu_from_case(fc)

@@ -84,7 +84,7 @@ def test_full_size_planes_by_residual():
     """1024^2, three fields, at the bench's settings: the oracle's own FP64
     residual of each GPU answer, and FP32+refine against FP64."""
     say("case full-size 1024^2 three fields")
-    fcs = cases.config2(n=1024)
+    fcs = cases.multifield_plane(n=1024)
     import math
 
     omega = 2.0 / (1.0 + math.sin(math.pi / 1024))
@@ -107,7 +107,7 @@ def test_full_size_planes_by_residual():
 def test_full_size_volume_by_residual():
     """256^3 spheroid: the oracle's FP64 residual of the FP32+refine answer."""
     say("case full-size 256^3 spheroid")
-    (fc,) = cases.config3()
+    (fc,) = cases.spheroid()
     from cellweave_amd import campaign
 
     g = gpu_from_case(fc)
