@@ -875,7 +875,7 @@ __global__ __launch_bounds__(NW * 64) void relax2d_reg(PassArgs a_in) {
     constexpr int RY = NW * NR;
     constexpr int TX = RX - 2 * H;
     constexpr int TY = RY - 2 * H;
-    static_assert(TY > 0 && NR >= 2 && S <= kMaxS * 2, "region shape");
+    static_assert(TY > 0 && NR >= 2 && S <= 16, "region shape");
     static_assert(!PLAIN || S <= kMaxS, "change slots");
     using T2 = typename V2<T>::type;
     __shared__ T edge[2][NW][2][2][64];  // [slot][wave][first/last row][colour][lane]

@@ -92,7 +92,8 @@ constexpr Shape kSmoothShapes[] = {{1, 32, 4, 0},   {2, 32, 4, 0},   {4, 64, 8, 
                                    {8, 128, 8, 0},  {8, 128, 16, 0}, {2, 32, 4, 1},   {4, 64, 16, 1},
                                    {4, 64, 8, 1},   {8, 128, 16, 1}, {2, 64, 8, 1},   {4, 128, 16, 1},
                                    {2, 32, 4, 2},   {4, 64, 8, 2},   {4, 64, 16, 2},  {8, 128, 16, 2},
-                                   {2, 64, 8, 2},   {4, 128, 16, 2}};
+                                   {2, 64, 8, 2},   {4, 128, 16, 2}, {6, 128, 16, 2},
+                                   {10, 128, 16, 2}, {12, 128, 16, 2}};
 constexpr int kNumPlainShapes = (int)(sizeof(kPlainShapes) / sizeof(kPlainShapes[0]));
 constexpr int kNumSmoothShapes = (int)(sizeof(kSmoothShapes) / sizeof(kSmoothShapes[0]));
 // 3D shapes: kind 0 = in-place red and black half sweeps (half3d, two
@@ -125,6 +126,42 @@ int default_smooth3d_shape(cw_field* const* F, uint32_t nf) {
     return tiles < 384 ? kVolumeChunk8 : kVolumeChunk16;
 }
 int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 : 15; }
+
+// FP32 planes with no sweep count asked for: each relax2d_reg workgroup
+// (one per CU, 119 VGPRs a lane) loads its tile, about 14 us at full
+// occupancy, then sweeps it S times at about 0.9 us a sweep, and the launch
+// lasts as many of those rounds as it takes to run every tile. A deeper tile
+// spreads the load over more sweeps but leaves a smaller interior, so more
+// tiles; pick the S with the least time per sweep over whole rounds.
+constexpr int kPlaneDeepShapes[] = {15, 19, 20};  // S = 8, 10, 12 on 128 x 128 regions
+constexpr double kTileLoadUs = 14.0, kTileSweepUs = 0.9;
+int default_smooth2d_shape(cw_field* const* F, uint32_t nf) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    int best = kPlaneDeepShapes[0];
+    double best_us = 0.0;
+    for (int si : kPlaneDeepShapes) {
+        const Shape sh = kSmoothShapes[si];
+        const int tx = 128 - 4 * sh.s, ty = sh.ry - 4 * sh.s;
+        long tiles = 0;
+        for (uint32_t i = 0; i < nf; i++) {
+            const cw_field* f = F[i];
+            if (!f->has_span) continue;
+            const int sx = f->span[1] - f->span[0] + 1, sy = f->span[3] - f->span[2] + 1;
+            tiles += (long)((sx + tx - 1) / tx) * ((sy + ty - 1) / ty);
+        }
+        const long rounds = std::max(1L, (tiles + cus - 1) / cus);
+        const double us = rounds * (kTileLoadUs + sh.s * kTileSweepUs) / sh.s;
+        if (si == kPlaneDeepShapes[0] || us < best_us) {
+            best = si;
+            best_us = us;
+        }
+    }
+    return best;
+}
 
 }  // namespace
 
@@ -350,7 +387,10 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a) {
         case 14: CW_L((relax2d_reg<float, 4, 16, 4, false, false>), b1024); break;
         case 15: CW_L((relax2d_reg<float, 8, 16, 8, false, false>), b1024); break;
         case 16: CW_L((relax2d_reg<float, 2, 8, 8, false, false>), b512); break;
-        default: CW_L((relax2d_reg<float, 4, 16, 8, false, false>), b1024); break;
+        case 17: CW_L((relax2d_reg<float, 4, 16, 8, false, false>), b1024); break;
+        case 18: CW_L((relax2d_reg<float, 6, 16, 8, false, false>), b1024); break;
+        case 19: CW_L((relax2d_reg<float, 10, 16, 8, false, false>), b1024); break;
+        default: CW_L((relax2d_reg<float, 12, 16, 8, false, false>), b1024); break;
         }
     }
 #undef CW_L
@@ -892,7 +932,10 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     for (uint32_t i = 0; i < nf; i++)
         if ((F[i]->D > 1) != three_d)
             return fail(CW_ERR_ARG, "a batch mixes planes and volumes");
-    int shape = o.shape >= 0 ? o.shape : default_smooth_shape(o.s ? o.s : 8);
+    int shape = o.shape >= 0 ? o.shape
+                : o.s      ? default_smooth_shape(o.s)
+                : three_d  ? 0
+                           : default_smooth2d_shape(F, nf);
     if (!three_d && shape >= kNumSmoothShapes)
         return fail(CW_ERR_ARG, "unknown tile shape %d", shape);
     if (three_d && o.shape >= kNum3DShapes)

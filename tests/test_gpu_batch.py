@@ -67,3 +67,26 @@ def test_small_campaign_converges_and_matches_single_solves(precision):
             assert st["omega"] == stats[k]["omega"]
             assert st["sweeps"] == stats[k]["sweeps"]
             assert bits_equal(g.download(), fields[k].download())
+
+
+def test_plane_smoothing_depth_follows_workgroup_rounds():
+    """With no sweep count asked for, FP32 planes take the temporal blocking
+    depth with the least time per sweep over whole rounds of one workgroup
+    per CU: 12 for three 1024^2 fields (507 tiles, two rounds either way),
+    8 for one free 2048^2 field (484 tiles fill two rounds, deeper tiles
+    need three). MI355X has 256 CUs."""
+    from cellweave_amd.field import Field
+
+    say("case smoothing depth 3 x 1024^2")
+    fcs = cases.multifield_plane(n=1024)
+    gs = [gpu_from_case(fc) for fc in fcs]
+    st, _ = batch_relax(gs, criterion=CRIT_REL_RESIDUAL, tolerance=1e-3, omega=1.99,
+                        precision=PREC_FP32_REFINE)
+    assert [s["sweeps_per_pass"] for s in st] == [12, 12, 12]
+    say("case smoothing depth 2048^2")
+    g = Field(2048, 2048, 1, 0.3, 0.05, 0.0)
+    g.set_source(2e-3)
+    s = g.relax_ex(omega=1.0, criterion=CRIT_REL_RESIDUAL, tolerance=1e-6,
+                   precision=PREC_FP32_REFINE, max_sweeps=100_000)
+    assert s["sweeps_per_pass"] == 8
+    assert np.max(np.abs(g.download() - 2e-3 / 0.05)) <= 1e-5 * (2e-3 / 0.05)
