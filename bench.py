@@ -105,6 +105,47 @@ def cpu_baseline(fc, budget_s=12.0, threads=None):
                       f"({span} swept voxels), {dt:.1f} s, CPU restatement of pde.rs"}
 
 
+def cpu_tissue_steps(gpu_sweeps, steps=4, threads=None):
+    """The same lattice-step loop on the oracle (C restatement of pde.rs and
+    coupling.rs, OpenMP over rows): the warm-up solve, then the first `steps`
+    lattice steps, each timed. The sweep counts sit beside the GPU's."""
+    import numpy as np
+
+    from cellweave_amd import cases, workloads as wl
+    from oracle.orf import OracleField
+
+    if threads is None:
+        threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
+    seq, ncells = wl.tissue_steps(steps=steps)
+    n = 120
+    consumes = np.ones(ncells + 1, np.uint8)
+    consumes[0] = 0
+    f = OracleField(n, n, n, cases.d_px(2000.0), 0.0, 1.0)
+
+    def declare(ids):
+        f.apply_uptake(ids, consumes, wl.TISSUE_UPTAKE)
+        f.release_all()
+        f.hold_bath_at(1.0, (ids == 0).astype(np.uint8))
+
+    declare(seq[0])
+    f.relax(*wl.TISSUE_WARMUP, threads=threads)
+    f.mean_per_cell(seq[0], ncells + 1)
+    ts, sweeps = [], []
+    for ids in seq[1:]:
+        t0 = time.perf_counter()
+        declare(ids)
+        sweeps.append(f.relax(wl.TISSUE_TOLERANCE * wl.TISSUE_UPTAKE, wl.TISSUE_MAX_SWEEPS,
+                              threads=threads))
+        f.mean_per_cell(ids, ncells + 1)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    return {"value": 1.0 / t, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"warm-up solve, then {len(ts)} lattice steps of the {n}^3 tissue "
+                      f"(median {1e3 * t:.0f} ms a step), CPU restatement of pde.rs and coupling.rs",
+            "sweeps_per_step": sweeps,
+            "same_sweeps_as_gpu": sweeps == list(gpu_sweeps[:len(sweeps)])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,7 +159,7 @@ def main():
     ap.add_argument("--time-launches", type=int, default=200)
     ap.add_argument("--workload", default="multifield1024",
                     help="multifield1024 (the default line) or oxygen200, spheroid256, campaign512, "
-                         "adjoint192, coupling")
+                         "adjoint192, coupling, tissue120")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: every GPU box "
@@ -131,7 +172,11 @@ def main():
     if args.workload != "multifield1024":
         from cellweave_amd import workloads
 
-        print(json.dumps(workloads.WORKLOADS[args.workload]()))
+        line = workloads.WORKLOADS[args.workload]()
+        if args.workload == "tissue120" and not args.no_cpu:
+            log("cpu baseline")
+            line["cpu_baseline"] = cpu_tissue_steps(line["sweeps_per_step"])
+        print(json.dumps(line))
         return
 
     from cellweave_amd import cases

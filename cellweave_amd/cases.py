@@ -254,3 +254,40 @@ def necrotic_mask(n: int, necrotic: float) -> np.ndarray:
     c = n / 2.0
     zz, yy, xx = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
     return ((xx - c) ** 2 + (yy - c) ** 2 + (zz - c) ** 2 < necrotic * necrotic).reshape(-1)
+
+
+def tissue_box(seed: int = 6000, n: int = 120, radius: float = 45.0):
+    """A ball of Voronoi cells in a bath: an n^3 box, cells at the spheroid's
+    density (~93 voxels a cell) inside a centred ball, medium (id 0) around it.
+    Returns (ids [n^3] u32, number of cells)."""
+    rng = np.random.default_rng(seed)
+    ncells = max(4, int(round(4.0 / 3.0 * math.pi * radius ** 3 / 93.2)))
+    return voronoi_ball(n, n / 2.0, radius, ncells, rng), ncells
+
+
+def lattice_copies(ids: np.ndarray, w: int, h: int, d: int, count: int, rng) -> np.ndarray:
+    """One lattice step in the manner of a Cellular Potts sweep: `count`
+    accepted copies, each a voxel taking the id of one of its face neighbours
+    where the two differ (cell-cell and cell-medium boundaries move), applied
+    in order. Returns the new ids; the input is not changed."""
+    out = ids.copy()
+    n = w * h * d
+    strides = [(1, w), (w, h), (w * h, d)]
+    done = 0
+    while done < count:
+        m = 4 * (count - done) + 64
+        v = rng.integers(0, n, m)
+        axis = rng.integers(0, 3, m)
+        sign = rng.integers(0, 2, m) * 2 - 1
+        step = np.array([s for s, _ in strides])[axis]
+        extent = np.array([e for _, e in strides])[axis]
+        coord = (v // step) % extent
+        ok = (coord + sign >= 0) & (coord + sign < extent)
+        v, nb = v[ok], (v + sign * step)[ok]
+        for a, b in zip(v.tolist(), nb.tolist()):
+            if out[a] != out[b]:
+                out[a] = out[b]
+                done += 1
+                if done == count:
+                    break
+    return out

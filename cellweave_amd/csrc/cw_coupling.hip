@@ -94,16 +94,19 @@ __global__ void k_scan_init(int* scan) {
     }
 }
 
-constexpr int kRedBlocks = 2048;  // grid of the reductions: grid-stride, one atomic per block
-
-// Reductions walk whole rows: block b takes rows b, b + grid, ...
+// Reductions walk whole rows: block b takes rows b, b + grid, ... and writes
+// its partial to slot b of the field's reduction slots; one block folds them.
+// (Thousands of blocks updating one cache line with atomics serialise at the
+// L2: 2048 blocks took ~140 us that way.)
 unsigned red_grid(const cw_field* f) {
     return (unsigned)std::min<size_t>(kRedBlocks, (size_t)f->H * f->D);
 }
 
-// Block-wide combine of (flags, box) and one set of atomics per block.
-__device__ inline void box_commit(int flags, int mnx, int mxx, int mny, int mxy, int mnz, int mxz,
-                                  int* out_flags, int* out_box) {
+int* red_slots(const cw_field* f) { return f->scan + kScanInts; }
+
+// Block-wide combine of (flags, box); thread 0 gets the block's result.
+__device__ inline void box_reduce(int& flags, int& mnx, int& mxx, int& mny, int& mxy, int& mnz,
+                                  int& mxz) {
     __shared__ int sh[kBlock / 64][7];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     flags = wave_or(flags);
@@ -122,19 +125,44 @@ __device__ inline void box_commit(int flags, int mnx, int mxx, int mny, int mxy,
             mny = min(mny, sh[w][3]); mxy = max(mxy, sh[w][4]);
             mnz = min(mnz, sh[w][5]); mxz = max(mxz, sh[w][6]);
         }
-        if (flags) {
-            atomicOr(out_flags, flags);
-            if (mxx >= 0) {
-                atomicMin(&out_box[0], mnx); atomicMax(&out_box[1], mxx);
-                atomicMin(&out_box[2], mny); atomicMax(&out_box[3], mxy);
-                atomicMin(&out_box[4], mnz); atomicMax(&out_box[5], mxz);
-            }
+    }
+}
+
+__device__ inline void box_partial(int flags, int mnx, int mxx, int mny, int mxy, int mnz, int mxz,
+                                   int* slots) {
+    box_reduce(flags, mnx, mxx, mny, mxy, mnz, mxz);
+    if (threadIdx.x == 0) {
+        int* o = slots + (size_t)blockIdx.x * kRedSlot;
+        o[0] = flags; o[1] = mnx; o[2] = mxx; o[3] = mny; o[4] = mxy; o[5] = mnz; o[6] = mxz;
+    }
+}
+
+// One block: folds `nblocks` (flags, box) slots into out_flags and out_box,
+// which hold the identity (0, INT_MAX / -1 pairs) or earlier results.
+__global__ __launch_bounds__(kBlock) void k_box_fold(const int* slots, int nblocks,
+                                                     int* out_flags, int* out_box) {
+    int flags = 0;
+    int mnx = INT_MAX, mxx = -1, mny = INT_MAX, mxy = -1, mnz = INT_MAX, mxz = -1;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) {
+        const int* o = slots + (size_t)b * kRedSlot;
+        flags |= o[0];
+        mnx = min(mnx, o[1]); mxx = max(mxx, o[2]);
+        mny = min(mny, o[3]); mxy = max(mxy, o[4]);
+        mnz = min(mnz, o[5]); mxz = max(mxz, o[6]);
+    }
+    box_reduce(flags, mnx, mxx, mny, mxy, mnz, mxz);
+    if (threadIdx.x == 0 && flags) {
+        *out_flags |= flags;
+        if (mxx >= 0) {
+            out_box[0] = min(out_box[0], mnx); out_box[1] = max(out_box[1], mxx);
+            out_box[2] = min(out_box[2], mny); out_box[3] = max(out_box[3], mxy);
+            out_box[4] = min(out_box[4], mnz); out_box[5] = max(out_box[5], mxz);
         }
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_scan(Geo g, const uint8_t* held, const double* up,
-                                                 const double* pr, double source, int* scan) {
+                                                 const double* pr, double source, int* slots) {
     int flags = 0;
     int mnx = INT_MAX, mxx = -1, mny = INT_MAX, mxy = -1, mnz = INT_MAX, mxz = -1;
     const size_t rows = (size_t)g.H * g.D;
@@ -150,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_scan(Geo g, const uint8_t* held, con
         mny = min(mny, y); mxy = max(mxy, y);
         mnz = min(mnz, z); mxz = max(mxz, z);
     }
-    box_commit(flags, mnx, mxx, mny, mxy, mnz, mxz, &scan[10], &scan[4]);
+    box_partial(flags, mnx, mxx, mny, mxy, mnz, mxz, slots);
 }
 
 // k = p or -u (folded, exact when no voxel has both and source == 0) or
@@ -272,7 +300,7 @@ __global__ void k_box_init(int* scan) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_closed_box(Geo g, const uint8_t* open, int* scan) {
+__global__ __launch_bounds__(kBlock) void k_closed_box(Geo g, const uint8_t* open, int* slots) {
     int flags = 0;
     int mnx = INT_MAX, mxx = -1, mny = INT_MAX, mxy = -1, mnz = INT_MAX, mxz = -1;
     const size_t rows = (size_t)g.H * g.D;
@@ -286,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_closed_box(Geo g, const uint8_t* ope
         mny = min(mny, y); mxy = max(mxy, y);
         mnz = min(mnz, z); mxz = max(mxz, z);
     }
-    box_commit(flags, mnx, mxx, mny, mxy, mnz, mxz, &scan[0], &scan[1]);
+    box_partial(flags, mnx, mxx, mny, mxy, mnz, mxz, slots);
 }
 
 // Seeds: open, not yet held voxels on the outer layer of the box (its first
@@ -393,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_flood_finish(Geo g, Box b, int has_b
 
 // ---- reductions: free count and inflow (pde.rs:415-461) ----------------------
 __global__ __launch_bounds__(kBlock) void k_count_free(Geo g, const uint8_t* held,
-                                                       unsigned long long* out) {
+                                                       unsigned long long* slots) {
     __shared__ unsigned long long sh[kBlock / 64];
     unsigned long long v = 0;
     const size_t rows = (size_t)g.H * g.D;
@@ -404,7 +432,21 @@ __global__ __launch_bounds__(kBlock) void k_count_free(Geo g, const uint8_t* hel
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < kBlock / 64; w++) v += sh[w];
-        if (v) atomicAdd(out, v);
+        slots[blockIdx.x] = v;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_u64(const unsigned long long* slots, int nblocks,
+                                                    unsigned long long* out) {
+    __shared__ unsigned long long sh[kBlock / 64];
+    unsigned long long v = 0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) v += slots[b];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kBlock / 64; w++) v += sh[w];
+        *out = v;
     }
 }
 
@@ -595,15 +637,25 @@ __global__ __launch_bounds__(kBlock) void k_cell_ranges(const uint32_t* keys, si
     if (j + 1 == n || keys[j + 1] != k) last[k] = (uint32_t)(j + 1);
 }
 
+// One wave per cell. The lanes gather 64 of the cell's values at a time and
+// the sum takes them in lane order, which is ascending voxel index (the CSR
+// lists come from a stable sort): the reference's sequential order, so the
+// same bits, without one lane walking a cell's voxels load after load.
 __global__ __launch_bounds__(kBlock) void k_cell_means(const double* c, const uint32_t* vals,
                                                        const uint32_t* first, const uint32_t* last,
                                                        uint32_t ncells, double* out) {
-    const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-    if (k >= ncells) return;
+    const uint32_t k = (uint32_t)(((size_t)blockIdx.x * kBlock + threadIdx.x) >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= ncells) return;  // whole waves
     const uint32_t a = first[k], b = last[k];
     double s = 0.0;
-    for (uint32_t j = a; j < b; j++) s += c[vals[j]];
-    out[k] = b > a ? s / (double)(b - a) : 0.0;
+    for (uint32_t j0 = a; j0 < b; j0 += 64) {
+        const uint32_t j = j0 + (uint32_t)lane;
+        const double v = j < b ? c[vals[j]] : 0.0;
+        const int m = (int)min(64u, b - j0);
+        for (int l = 0; l < m; l++) s += __shfl(v, l, 64);
+    }
+    if (lane == 0) out[k] = b > a ? s / (double)(b - a) : 0.0;
 }
 
 // ---- helpers ------------------------------------------------------------------
@@ -662,7 +714,9 @@ cw_status flood_hold(cw_field* f, const uint8_t* open, uint8_t* reach, double va
     const unsigned gb = grid_of(f->npad);
     hipLaunchKernelGGL(k_box_init, dim3(1), dim3(64), 0, f->stream, f->scan);
     hipLaunchKernelGGL(k_closed_box, dim3(red_grid(f)), dim3(kBlock), 0, f->stream, g, open,
-                       f->scan);
+                       red_slots(f));
+    hipLaunchKernelGGL(k_box_fold, dim3(1), dim3(kBlock), 0, f->stream, (const int*)red_slots(f),
+                       (int)red_grid(f), f->scan, f->scan + 1);
     int box[8];
     CW_HIP(hipMemcpyAsync(box, f->scan, sizeof box, hipMemcpyDeviceToHost, f->stream));
     CW_HIP(hipStreamSynchronize(f->stream));
@@ -775,7 +829,9 @@ cw_status build_coefficients(cw_field* f) {
     hipLaunchKernelGGL(k_scan_init, dim3(1), dim3(64), 0, f->stream, f->scan);
     hipLaunchKernelGGL(k_scan, dim3(red_grid(f)), dim3(kBlock), 0, f->stream, g,
                        (const uint8_t*)f->held,
-                       (const double*)f->up, (const double*)f->pr, f->source, f->scan);
+                       (const double*)f->up, (const double*)f->pr, f->source, red_slots(f));
+    hipLaunchKernelGGL(k_box_fold, dim3(1), dim3(kBlock), 0, f->stream, (const int*)red_slots(f),
+                       (int)red_grid(f), f->scan + 10, f->scan + 4);
     int raw[11];
     CW_HIP(hipMemcpyAsync(raw, f->scan, sizeof raw, hipMemcpyDeviceToHost, f->stream));
     CW_HIP(hipStreamSynchronize(f->stream));
@@ -961,9 +1017,11 @@ cw_status cw_field_hold_medium_bath(cw_field* f, cw_lattice* l, double value) {
 cw_status cw_field_free_count(cw_field* f, uint64_t* out) {
     if (!f || !out) return fail(CW_ERR_ARG, "null argument");
     unsigned long long* dev = (unsigned long long*)(f->scan + 32);
-    CW_HIP(hipMemsetAsync(dev, 0, sizeof *dev, f->stream));
+    unsigned long long* slots = (unsigned long long*)red_slots(f);
     hipLaunchKernelGGL(k_count_free, dim3(red_grid(f)), dim3(kBlock), 0, f->stream,
-                       geo_of(f), (const uint8_t*)f->held, dev);
+                       geo_of(f), (const uint8_t*)f->held, slots);
+    hipLaunchKernelGGL(k_sum_u64, dim3(1), dim3(kBlock), 0, f->stream,
+                       (const unsigned long long*)slots, (int)red_grid(f), dev);
     CW_HIP(hipGetLastError());
     unsigned long long v = 0;
     CW_HIP(hipMemcpyAsync(&v, dev, sizeof v, hipMemcpyDeviceToHost, f->stream));
@@ -1022,7 +1080,7 @@ cw_status cw_field_mean_per_cell(cw_field* f, cw_lattice* l, uint32_t ncells, do
     if (ncells == 0) return CW_OK;
     if ((st = build_csr(l, ncells))) return st;
     if ((st = stage_tables(l, f->stream, nullptr, nullptr, ncells))) return st;
-    hipLaunchKernelGGL(k_cell_means, dim3(grid_of(ncells)), dim3(kBlock), 0, f->stream,
+    hipLaunchKernelGGL(k_cell_means, dim3(grid_of((size_t)ncells * 64)), dim3(kBlock), 0, f->stream,
                        (const double*)f->c[f->cur], (const uint32_t*)l->vals[1],
                        (const uint32_t*)l->first, (const uint32_t*)l->last, ncells, l->table);
     CW_HIP(hipGetLastError());

@@ -31,6 +31,12 @@ constexpr uint32_t kObstacle = 0xFFFFFFFFu;  // cpm.rs:137 OBSTACLE = u32::MAX
                              hipGetErrorString(e_), __FILE__, __LINE__);                 \
     } while (0)
 
+// Grid-stride reductions run at most kRedBlocks workgroups; each writes its
+// partial result to its own slot and one workgroup folds the slots.
+constexpr int kScanInts = 64;
+constexpr int kRedBlocks = 2048;
+constexpr int kRedSlot = 8;
+
 // Device mirror of ScalarField (crates/engine/src/pde.rs:30-53). Every
 // per-voxel array is pitched: row pitch P = ceil(W/64)*64 elements, plane
 // stride P*H, x fastest. The device copies of held/uptake/production are the
@@ -52,7 +58,8 @@ struct cw_field {
     float* e[2] = {nullptr, nullptr};
     float* r32 = nullptr;
     float* lb = nullptr;
-    int* scan = nullptr;       // small device scratch of the coefficient scan
+    int* scan = nullptr;       // kScanInts of small results, then kRedBlocks slots of kRedSlot
+                               // ints: per-workgroup partials of the reductions
     mutable void* stage = nullptr;  // flat device staging of host copies
     mutable size_t stage_bytes = 0;
     bool has_span = false;

@@ -286,7 +286,13 @@ __device__ __forceinline__ void block_moved(Ctl& c, int pass, const unsigned lon
     if (threadIdx.x < S) {
         unsigned long long m = 0ull;
         for (int w = 0; w < NW; w++) m = s_m[w][threadIdx.x] > m ? s_m[w][threadIdx.x] : m;
-        if (m) atomicMax(&c.moved[pass % 3][threadIdx.x], m);
+        // Thousands of workgroups' atomics on one word serialise. The slot
+        // only grows during a launch, so a workgroup whose largest change is
+        // not above the slot's current value (a coherent load) changes
+        // nothing and skips its atomic: the result is the same maximum.
+        unsigned long long* slot = &c.moved[pass % 3][threadIdx.x];
+        if (m && m > __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(slot, m);
     }
 }
 

@@ -102,28 +102,41 @@ constexpr int kNumSmoothShapes = (int)(sizeof(kSmoothShapes) / sizeof(kSmoothSha
 struct Shape3 { int kind, nw, nr, zc; };
 constexpr Shape3 k3DShapes[] = {{0, 4, 1, 1},  {1, 8, 4, 16}, {1, 4, 8, 16}, {1, 8, 4, 32},
                                  {1, 8, 4, 8},  {1, 8, 4, 24}, {1, 4, 8, 32}, {1, 8, 4, 48},
-                                 {1, 4, 8, 24}};
+                                 {1, 4, 8, 24}, {1, 8, 4, 4},  {1, 4, 8, 8},  {1, 4, 8, 4}};
 constexpr int kNum3DShapes = (int)(sizeof(k3DShapes) / sizeof(k3DShapes[0]));
 constexpr int kDefault3DShape = 1;
 
 int default_plain_shape(int s) { return s <= 1 ? 2 : 3; }
 
-// FP32 volumes: the z-streaming smoother is latency-bound, so workgroups
-// matter more than the halo planes a shorter chunk re-reads. 16-plane chunks
-// unless they leave fewer than 384 workgroups for the batch, then 8-plane.
-constexpr int kVolumeChunk16 = 1, kVolumeChunk8 = 4;
-int default_smooth3d_shape(cw_field* const* F, uint32_t nf) {
-    long tiles = 0;
-    const Shape3 sh = k3DShapes[kVolumeChunk16];
-    const int ty = sh.nw * sh.nr - 4;
-    for (uint32_t i = 0; i < nf; i++) {
-        const cw_field* f = F[i];
-        if (!f->has_span) continue;
-        const int sx = f->span[1] - f->span[0] + 1, sy = f->span[3] - f->span[2] + 1;
-        const int sz = f->span[5] - f->span[4] + 1;
-        tiles += (long)((sx + 123) / 124) * ((sy + ty - 1) / ty) * ((sz + sh.zc - 1) / sh.zc);
+// Volumes: the z-streaming kernels are latency-bound when workgroups are
+// few, so the halo planes a shorter chunk re-reads matter less than the
+// workgroups it adds. The longest chunk (16, 8 or 4 planes) that still gives
+// the batch 240 workgroups; failing that, spans under 2^18 voxels take the
+// in-place red and black half sweeps (an elementwise grid, two launches a
+// sweep) and larger ones 4-plane chunks. Measured per sweep on single fields
+// (in-place / 4 / 8 / 16 planes, us):
+//   FP64  49^3: 9.9 / 19 / 30 / 54    91^3: 90 / 21 / 35 / 62    151^3: 61 / 46 / 39 / 64
+//   FP32  49^3: 10 / 15 / 24 / 42     91^3: 16 / 16 / 25 / 43    151^3: 63 / 27 / 29 / 47
+constexpr int kVolumeInPlace = 0, kVolumeChunk16 = 1, kVolumeChunk8 = 4, kVolumeChunk4 = 9;
+int default_volume_shape(cw_field* const* F, uint32_t nf) {
+    const int chunks[3] = {kVolumeChunk16, kVolumeChunk8, kVolumeChunk4};
+    uint64_t voxels = 0;
+    for (int c = 0; c < 3; c++) {
+        const Shape3 sh = k3DShapes[chunks[c]];
+        const int ty = sh.nw * sh.nr - 4;
+        long tiles = 0;
+        voxels = 0;
+        for (uint32_t i = 0; i < nf; i++) {
+            const cw_field* f = F[i];
+            if (!f->has_span) continue;
+            const int sx = f->span[1] - f->span[0] + 1, sy = f->span[3] - f->span[2] + 1;
+            const int sz = f->span[5] - f->span[4] + 1;
+            tiles += (long)((sx + 123) / 124) * ((sy + ty - 1) / ty) * ((sz + sh.zc - 1) / sh.zc);
+            voxels += (uint64_t)sx * sy * sz;
+        }
+        if (tiles >= 240) return chunks[c];
     }
-    return tiles < 384 ? kVolumeChunk8 : kVolumeChunk16;
+    return voxels < (1u << 18) ? kVolumeInPlace : kVolumeChunk4;
 }
 int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 : 15; }
 
@@ -403,21 +416,21 @@ cw_status launch_stream3d(Plan& pl, const PassArgs& a) {
     if (nt == 0) return CW_OK;
     const dim3 grid(nt), b256(256), b512(512);
     hipStream_t st = pl.stream;
-    const int sh = pl.shape3;
+    const bool w4 = k3DShapes[pl.shape3].nw == 4;  // 4 waves of 8 rows, else 8 of 4
 #define CW_L(kern, blk) hipLaunchKernelGGL(kern, grid, blk, 0, st, a)
     if (pl.fp64 && pl.plain) {
         if (pl.general) {
-            if (sh == 2) CW_L((relax3d_stream<double, 4, 8, true, true>), b256);
+            if (w4) CW_L((relax3d_stream<double, 4, 8, true, true>), b256);
             else CW_L((relax3d_stream<double, 8, 4, true, true>), b512);
         } else {
-            if (sh == 2) CW_L((relax3d_stream<double, 4, 8, true, false>), b256);
+            if (w4) CW_L((relax3d_stream<double, 4, 8, true, false>), b256);
             else CW_L((relax3d_stream<double, 8, 4, true, false>), b512);
         }
     } else if (pl.fp64) {
-        if (sh == 2) CW_L((relax3d_stream<double, 4, 8, false, false>), b256);
+        if (w4) CW_L((relax3d_stream<double, 4, 8, false, false>), b256);
         else CW_L((relax3d_stream<double, 8, 4, false, false>), b512);
     } else {
-        if (sh == 2) CW_L((relax3d_stream<float, 4, 8, false, false>), b256);
+        if (w4) CW_L((relax3d_stream<float, 4, 8, false, false>), b256);
         else CW_L((relax3d_stream<float, 8, 4, false, false>), b512);
     }
 #undef CW_L
@@ -773,7 +786,8 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     pl.use_graph = o.use_graph;
     if (three_d) {
         // volumes: tile_shape picks a 3D shape, one sweep per launch
-        pl.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape : kDefault3DShape;
+        pl.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape
+                                                               : default_volume_shape(F, nf);
         pl.shape = 0;
         pl.s_main = 1;
     }
@@ -953,7 +967,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     inner.use_graph = o.use_graph;
     if (three_d)
         inner.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape
-                                                                 : default_smooth3d_shape(F, nf);
+                                                                 : default_volume_shape(F, nf);
     std::vector<int> idx;
     std::vector<double> omegas(nf);
     int longest = 2;
@@ -1390,7 +1404,9 @@ cw_status cw_field_create(uint32_t w, uint32_t h, uint32_t d, double diff, doubl
     if ((st = alloc_zero((void**)&f->up, b8, f->stream, 0))) return bail(st);
     if ((st = alloc_zero((void**)&f->pr, b8, f->stream, 0))) return bail(st);
     if ((st = alloc_zero((void**)&f->held, f->npad, f->stream, 1))) return bail(st);
-    if ((st = alloc_zero((void**)&f->scan, 64 * sizeof(int), f->stream, 0))) return bail(st);
+    if ((st = alloc_zero((void**)&f->scan, (kScanInts + kRedBlocks * kRedSlot) * sizeof(int),
+                         f->stream, 0)))
+        return bail(st);
     // nothing held inside the box; the padding stays held
     CW_HIP(hipMemset2DAsync(f->held, f->P, 0, f->W, (size_t)f->H * f->D, f->stream));
     {

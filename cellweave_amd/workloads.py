@@ -10,6 +10,7 @@ dominant kernel for the roofline.
     campaign512:  512 histories x 2 fields, batched per size class, graph replays
     adjoint192:   192^3 adjoint, gradients against central finite differences
     coupling:     the device-side gathers, holds, step and per-cell means
+    tissue120:    the per-lattice-step loop (sources, warm-started relax, means) at 120^3
 """
 from __future__ import annotations
 
@@ -389,5 +390,81 @@ def coupling(reps=10):
                      "closed-box scan, a tiled flood to a fixed point and a finish pass"}
 
 
+TISSUE_UPTAKE = 1e-3        # per s, every living cell consumes
+TISSUE_TOLERANCE = 0.01     # per-step tolerance, times the uptake (simulation.rs:85)
+TISSUE_MAX_SWEEPS = 20_000  # simulation.rs:88
+TISSUE_WARMUP = (1e-8, 2_000_000)  # simulation.rs:95, :98
+
+
+def tissue_steps(n=120, radius=45.0, steps=10, copies=3800, seed=6000):
+    """The inputs of the lattice-step workload: the first lattice and one
+    lattice per step (each `copies` boundary moves from the last), all
+    seeded. Returns (ids per step with the first at index 0, cell count)."""
+    ids, ncells = cases.tissue_box(seed=seed, n=n, radius=radius)
+    rng = np.random.default_rng(seed + 1)
+    seq = [ids]
+    for _ in range(steps):
+        seq.append(cases.lattice_copies(seq[-1], n, n, n, copies, rng))
+    return seq, ncells
+
+
+def tissue120(n=120, radius=45.0, steps=10, copies=3800):
+    """The reference's per-step loop on a 120^3 tissue in a bath, with every
+    array on the device: a warm-up solve to 1e-8 absolute (simulation.rs:1068-1083),
+    then per lattice step the lattice upload, declare_sources (uptake of the
+    living cells, release, the medium bath held at 1; simulation.rs:2138-2145),
+    the warm-started relax to 0.01 x uptake with at most 20,000 sweeps
+    (simulation.rs:1144-1154) and the per-cell means (simulation.rs:1157).
+    FP64 plain with the reference's stopping rule: the same sweeps and bits as
+    the CPU loop. Metric: lattice steps/s; the values download that a host
+    reader of every value would add is timed apart."""
+    from .field import Lattice
+
+    seq, ncells = tissue_steps(n, radius, steps, copies)
+    consumes = np.ones(ncells + 1, np.uint8)
+    consumes[0] = 0
+    lat = Lattice(n, n, n)
+    f = Field(n, n, n, cases.d_px(2000.0), 0.0, 1.0)
+
+    def declare(ids):
+        lat.upload(ids)
+        f.apply_uptake(lat, consumes, TISSUE_UPTAKE)
+        f.release_all()
+        f.hold_medium_bath(lat, 1.0)
+
+    t0 = time.perf_counter()
+    declare(seq[0])
+    warm = f.relax(*TISSUE_WARMUP)
+    f.mean_per_cell(lat, ncells + 1)
+    t_warm = time.perf_counter() - t0
+    per_step, sweeps, dl = [], [], []
+    for ids in seq[1:]:
+        t0 = time.perf_counter()
+        declare(ids)
+        sweeps.append(f.relax(TISSUE_TOLERANCE * TISSUE_UPTAKE, TISSUE_MAX_SWEEPS))
+        means = f.mean_per_cell(lat, ncells + 1)
+        per_step.append(time.perf_counter() - t0)
+        t1 = time.perf_counter()
+        f.download()
+        dl.append(time.perf_counter() - t1)
+    t = float(np.median(per_step))
+    span = n * n * n
+    return {
+        "metric": "lattice steps/s (declare sources + warm-started relax + per-cell means)",
+        "value": 1.0 / t, "unit": "steps/s", "n_gpus": 1, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Voronoi ball in a bath, seeded boundary copies per step)",
+        "config": {"workload": f"{n}^3 tissue in a bath, {ncells} cells in a radius-{radius:g} "
+                               f"ball, {copies} boundary copies per step, {steps} steps",
+                   "tolerance": TISSUE_TOLERANCE * TISSUE_UPTAKE,
+                   "max_sweeps": TISSUE_MAX_SWEEPS, "precision": "fp64, reference rule"},
+        "ms_per_step": 1e3 * t, "ms_per_step_each": [1e3 * x for x in per_step],
+        "sweeps_per_step": sweeps, "warmup": {"sweeps": warm, "seconds": t_warm},
+        "voxel_sweeps_per_s": span * float(np.sum(sweeps)) / float(np.sum(per_step)),
+        "values_download_ms": 1e3 * float(np.median(dl)),
+        "mean_of_means": float(np.mean(means[1:])),
+    }
+
+
 WORKLOADS = {"oxygen200": oxygen200, "spheroid256": spheroid256, "campaign512": campaign512,
-             "adjoint192": adjoint192, "coupling": coupling}
+             "adjoint192": adjoint192, "coupling": coupling, "tissue120": tissue120}

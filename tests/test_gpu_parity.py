@@ -203,7 +203,7 @@ def test_fp32_refine_reaches_residual_oxygen_200(shape):
     assert st["outer_iters"] >= 1
 
 
-SHAPES_3D = range(9)  # in-place half sweeps (0) and z-streaming variants (1-8)
+SHAPES_3D = range(12)  # in-place half sweeps (0) and z-streaming variants (1-11)
 
 
 def ball_case(side=31, r=10.0, u=0.006, general=False):
