@@ -304,8 +304,11 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": f"relax2d_reg<float,{spp_used},...> (FP32 smoother, {spp_used} sweeps per launch)",
-            "alg_bytes_per_launch": bytes32, "launch_ms": ms32,
+            "kernel": f"relax2d_reg<float,{spp_used},...> (FP32 smoother, {spp_used} sweeps per launch; "
+                      f"one launch per field, the {len(fcs)} fields' launches concurrent on their "
+                      "own streams)",
+            "concurrent_launches": len(fcs),
+            "alg_bytes_all_fields_per_launch": bytes32, "launch_ms": ms32,
             # an unblocked smoother (one sweep per HBM pass) would have to
             # stream this many GB/s to sweep as fast
             "sweeps_per_launch": spp_used, "one_sweep_per_pass_equivalent_GBs": achieved * spp_used,

@@ -90,3 +90,43 @@ def test_plane_smoothing_depth_follows_workgroup_rounds():
                    precision=PREC_FP32_REFINE, max_sweeps=100_000)
     assert s["sweeps_per_pass"] == 8
     assert np.max(np.abs(g.download() - 2e-3 / 0.05)) <= 1e-5 * (2e-3 / 0.05)
+
+
+@pytest.mark.parametrize("precision", [PREC_FP64, PREC_FP32_REFINE])
+def test_a_few_fields_solve_concurrently_with_their_own_bits(precision):
+    """Two to four fields run as concurrent per-field solves on their own
+    streams: each gets the sweeps and bits it gets when solved alone."""
+    say(f"case concurrent batch precision={precision}")
+    fcs = cases.multifield_plane(n=160)
+    together = [gpu_from_case(fc) for fc in fcs]
+    st_t, _ = batch_relax(together, criterion=CRIT_REL_RESIDUAL, tolerance=1e-9, omega=1.95,
+                          precision=precision)
+    for fc, g, s in zip(fcs, together, st_t):
+        alone = gpu_from_case(fc)
+        (s1,), _ = batch_relax([alone], criterion=CRIT_REL_RESIDUAL, tolerance=1e-9, omega=1.95,
+                               precision=precision)
+        assert s["status"] == 0 and s1["status"] == 0
+        assert s["sweeps"] == s1["sweeps"]
+        assert bits_equal(g.download(), alone.download())
+
+
+def test_a_larger_batch_keeps_one_launch_sequence_and_the_same_bits():
+    say("case five-field batch")
+    fcs = cases.multifield_plane(n=128) + cases.multifield_plane(seed=2100, n=96)[:2]
+    together = [gpu_from_case(fc) for fc in fcs]
+    st_t, _ = batch_relax(together, criterion=CRIT_MAX_CHANGE, tolerance=1e-10, omega=1.9)
+    for fc, g, s in zip(fcs, together, st_t):
+        alone = gpu_from_case(fc)
+        n = alone.relax_ex(criterion=CRIT_MAX_CHANGE, tolerance=1e-10, omega=1.9)["sweeps"]
+        assert s["sweeps"] == n
+        assert bits_equal(g.download(), alone.download())
+
+
+def test_a_batch_may_not_mix_planes_and_volumes():
+    from cellweave_amd.field import Field
+
+    say("case mixed batch")
+    a = Field(16, 16, 1, 0.2, 0.01, 0.5)
+    b = Field(8, 8, 8, 0.2, 0.01, 0.5)
+    with pytest.raises(ValueError, match="mixes planes and volumes"):
+        batch_relax([a, b], criterion=CRIT_REL_RESIDUAL, tolerance=1e-8)
