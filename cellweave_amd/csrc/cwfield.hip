@@ -586,8 +586,14 @@ cw_status run_passes(Plan& pl, int crit, double tol, uint64_t budget, uint64_t d
     const int maxs = (int)std::min<uint64_t>(dev_max, 0x7fffffff);
     const uint64_t start = (uint64_t)pl.sweeps_before;
     long launched = 0;
-    int chunk = 4;
     const int S = pl.three_d ? 1 : pl.s_main;
+    // Without a stopping rule the run ends at the budget: enqueue all of it
+    // and read the verdicts once. With one, start small and double, so a
+    // warm start that settles in a few sweeps does not queue many no-op
+    // launches.
+    int chunk = crit == CRIT_NONE
+                    ? (int)std::min<uint64_t>((budget + (uint64_t)S - 1) / (uint64_t)S, 1u << 20)
+                    : 4;
     constexpr int kGraphPasses = 16;
     while (true) {
         // steady state: replay a captured chunk of passes
