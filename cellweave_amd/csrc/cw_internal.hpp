@@ -66,6 +66,9 @@ struct cw_field {
     int span[6] = {0, 0, 0, 0, 0, 0};
     bool has_sink = false;
     bool coef_dirty = true;
+    // Launch plans of this field's single-field solves, kept between calls so
+    // a warm-started relax allocates nothing (cwfield.hip owns the type).
+    void* plans = nullptr;
 };
 
 namespace cwi {

@@ -251,6 +251,25 @@ struct Plan {
     int g_k = 0, g_crit = -1, g_ce = 0, g_max = 0;
     double g_tol = 0.0;
 
+    // Back to a fresh plan, keeping the device buffers and their capacities
+    // (and dropping a captured graph, whose launch grids and arguments belong
+    // to the previous layout).
+    void reuse() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+        stream = nullptr;
+        nprob = 0; s_main = 1; shape = 0;
+        fp64 = true; plain = true; general = false; three_d = false;
+        shape3 = kDefault3DShape;
+        hp.clear(); hc.clear();
+        for (int i = 0; i < 3; i++) tp[i].clear();
+        part_stride = 0;
+        pass = 0; sweeps_before = 0; prev_s = 0; prev_tl = 0; prev_check = 0;
+        use_graph = false;
+        run_host[0] = run_host[1] = 0;
+        g_k = 0; g_crit = -1; g_ce = 0; g_max = 0; g_tol = 0.0;
+    }
+
     ~Plan() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         (void)hipFree(run);
@@ -268,6 +287,21 @@ cw_status grow(T** p, size_t* cap, size_t need) {
     CW_HIP(hipMalloc((void**)p, c * sizeof(T)));
     *cap = c;
     return CW_OK;
+}
+
+// A field's own plans for single-field solves: `main` for the FP64 sweep or
+// the refinement's outer (FP64) plan, `aux` for the refinement's FP32 inner
+// plan. Batches of several fields in one plan allocate their own.
+struct PlanCache {
+    Plan main, aux;
+};
+
+Plan& field_plan(cw_field* f, int which) {
+    if (!f->plans) f->plans = new PlanCache();
+    PlanCache* pc = (PlanCache*)f->plans;
+    Plan& p = which ? pc->aux : pc->main;
+    p.reuse();
+    return p;
 }
 
 // Lays out the tilings of every problem (s-sweep tile, one-sweep tile,
@@ -779,7 +813,8 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
             return fail(CW_ERR_ARG, "a batch mixes planes and volumes");
     int s = o.s ? std::min(o.s, 2) : 2;
     if (three_d) s = 1;
-    Plan pl;
+    Plan local;
+    Plan& pl = nf == 1 ? field_plan(F[0], 0) : local;
     pl.stream = stream;
     pl.fp64 = true;
     pl.three_d = three_d;
@@ -963,7 +998,9 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         return fail(CW_ERR_ARG, "unknown volume tile shape %d", o.shape);
     if (three_d) shape = 0;
     const int s = kSmoothShapes[shape].s;
-    Plan outer, inner;
+    Plan local_outer, local_inner;
+    Plan& outer = nf == 1 ? field_plan(F[0], 0) : local_outer;
+    Plan& inner = nf == 1 ? field_plan(F[0], 1) : local_inner;
     outer.stream = inner.stream = stream;
     outer.fp64 = true;
     inner.fp64 = false;
@@ -1493,6 +1530,7 @@ cw_status cw_field_create(uint32_t w, uint32_t h, uint32_t d, double diff, doubl
 cw_status cw_field_destroy(cw_field* f) {
     if (!f) return CW_OK;
     if (f->stream) hipStreamSynchronize(f->stream);
+    delete (PlanCache*)f->plans;
     hipFree(f->c[0]); hipFree(f->c[1]); hipFree(f->k); hipFree(f->k2);
     hipFree(f->up); hipFree(f->pr); hipFree(f->held); hipFree(f->scan); hipFree(f->stage);
     hipFree(f->e[0]); hipFree(f->e[1]); hipFree(f->r32); hipFree(f->lb);
