@@ -278,7 +278,9 @@ class Field:
 
 
 def batch_relax(fields, check=True, **kw):
-    """Solve every field in one launch sequence; returns per-field stats."""
+    """Solve every field; returns per-field stats. Two to four fields run as
+    concurrent solves on their own streams, more in one launch sequence;
+    each field gets the same bits either way."""
     o = make_opts(**kw)
     n = len(fields)
     arr = (C.c_void_p * n)(*[f._h.value for f in fields])
@@ -290,8 +292,10 @@ def batch_relax(fields, check=True, **kw):
 
 
 def time_passes(fields, passes, **kw) -> float:
-    """Average milliseconds of one sweep-kernel launch of the plan
-    cw_batch_relax would build (HIP events on the library stream)."""
+    """Average milliseconds of one sweep-kernel launch as cw_batch_relax
+    would run it (HIP events on the launching stream). With two to four
+    fields, their launches run concurrently and this is the longest
+    per-field average interval."""
     o = make_opts(**kw)
     n = len(fields)
     arr = (C.c_void_p * n)(*[f._h.value for f in fields])

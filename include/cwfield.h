@@ -138,10 +138,14 @@ cw_status cw_field_relax_ex(cw_field *field, const cw_relax_opts *opts,
 /* FP64 relative residual (see CW_CRIT_REL_RESIDUAL) of the device values. */
 cw_status cw_field_residual(cw_field *field, double *rel_residual);
 
-/* Batched solve (new; the campaign of src/campaign.rs:196-204 run as one
- * launch sequence): every field is swept by the same kernel launches, each
- * stops on its own criterion. stats (may be NULL) has `count` entries.
- * Returns CW_ERR_NOT_CONVERGED if any field did not converge. */
+/* Batched solve (new; the campaign of src/campaign.rs:196-204 run together).
+ * Five or more fields: one launch sequence, every field swept by the same
+ * kernel launches. Two to four fields on distinct streams: concurrent solves,
+ * one internal thread and the field's own stream each. Either way each field
+ * stops on its own criterion and gets the bits it gets when solved alone.
+ * stats (may be NULL) has `count` entries. Returns CW_ERR_NOT_CONVERGED if
+ * any field did not converge. The fields must not be in use elsewhere during
+ * the call. */
 cw_status cw_batch_relax(cw_field *const *fields, uint32_t count, const cw_relax_opts *opts,
                          cw_relax_stats *stats);
 
@@ -181,8 +185,9 @@ cw_status cw_autotune(cw_field *const *fields, uint32_t count, const cw_relax_op
 
 /* Measurement helper (no reference counterpart): builds the same launch plan
  * cw_batch_relax would for `opts` and times `passes` back-to-back launches of
- * its sweep kernel with HIP events on the library's stream. *ms_per_pass gets
- * the average. The fields' values are swept (not solved) by the call. */
+ * its sweep kernel with HIP events on the launching stream. *ms_per_pass gets
+ * the average; with two to four fields, whose launches run concurrently, the
+ * longest per-field average. The fields' values are swept (not solved). */
 cw_status cw_batch_time_passes(cw_field *const *fields, uint32_t count,
                                const cw_relax_opts *opts, uint32_t passes,
                                double *ms_per_pass);

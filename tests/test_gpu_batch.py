@@ -130,3 +130,26 @@ def test_a_batch_may_not_mix_planes_and_volumes():
     b = Field(8, 8, 8, 0.2, 0.01, 0.5)
     with pytest.raises(ValueError, match="mixes planes and volumes"):
         batch_relax([a, b], criterion=CRIT_REL_RESIDUAL, tolerance=1e-8)
+
+
+def test_a_few_volumes_solve_concurrently_with_their_own_bits():
+    say("case concurrent volumes")
+    fcs = [cases.spheroid(seed=3100 + k, n=36 + 4 * k, ncells=40, radius=12.0 + k, necrotic=3.0)[0]
+           for k in range(3)]
+    together = [gpu_from_case(fc) for fc in fcs]
+    st_t, _ = batch_relax(together, criterion=CRIT_MAX_CHANGE, tolerance=1e-10, omega=1.8)
+    for fc, g, s in zip(fcs, together, st_t):
+        alone = gpu_from_case(fc)
+        n = alone.relax_ex(criterion=CRIT_MAX_CHANGE, tolerance=1e-10, omega=1.8)["sweeps"]
+        assert s["sweeps"] == n
+        assert bits_equal(g.download(), alone.download())
+
+
+def test_a_concurrent_batch_reports_a_field_that_does_not_settle():
+    from cellweave_amd.field import SimulationError
+
+    say("case concurrent batch, one field capped")
+    fcs = cases.multifield_plane(n=96)
+    gs = [gpu_from_case(fc) for fc in fcs]
+    with pytest.raises(SimulationError, match="did not settle"):
+        batch_relax(gs, criterion=CRIT_REL_RESIDUAL, tolerance=1e-12, omega=1.9, max_sweeps=40)
