@@ -932,6 +932,10 @@ __global__ __launch_bounds__(NW * 64) void relax2d_reg(PassArgs a_in) {
         if (has_k2) lu[i] = *(const T2*)(k2p + off);
         if (!replay) lv[i] = *(const T2*)(xin + off);
     }
+    // Decide whether this launch runs while the tile loads are in flight: the
+    // verdict's own reads then overlap them instead of following them.
+    int par;
+    if (!sweep_prologue(a, pr, p, lt, &par)) return;
 #pragma unroll
     for (int i = 0; i < NR; i++) {
         const int gy = gyo + r0 + i;
@@ -961,8 +965,6 @@ __global__ __launch_bounds__(NW * 64) void relax2d_reg(PassArgs a_in) {
         }
     }
 
-    int par;
-    if (!sweep_prologue(a, pr, p, lt, &par)) return;
     const T* src = (const T*)pr.x[par];
     T* dst = (T*)pr.x[par ^ 1];
     if (replay) {
