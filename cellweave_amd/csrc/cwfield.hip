@@ -250,6 +250,10 @@ struct Plan {
     hipGraphExec_t gexec = nullptr;
     int g_k = 0, g_crit = -1, g_ce = 0, g_max = 0;
     double g_tol = 0.0;
+    // run_passes lays the plan out again over its unstopped problems
+    bool compact = false;
+    std::vector<char> act;
+    int nact = 0;
 
     // Back to a fresh plan, keeping the device buffers and their capacities
     // (and dropping a captured graph, whose launch grids and arguments belong
@@ -268,6 +272,7 @@ struct Plan {
         use_graph = false;
         run_host[0] = run_host[1] = 0;
         g_k = 0; g_crit = -1; g_ce = 0; g_max = 0; g_tol = 0.0;
+        compact = false; act.clear(); nact = 0;
     }
 
     ~Plan() {
@@ -306,15 +311,26 @@ Plan& field_plan(cw_field* f, int which) {
 
 // Lays out the tilings of every problem (s-sweep tile, one-sweep tile,
 // elementwise tile) and uploads problems, control blocks and tables.
-cw_status plan_upload(Plan& pl) {
+// With `active`, problems marked 0 get no tiles: launches skip them without
+// dispatching a workgroup. A captured graph is dropped, since its launch
+// grids follow the tile lists.
+cw_status plan_upload(Plan& pl, const std::vector<char>* active = nullptr) {
     const int np = pl.nprob;
+    if (pl.gexec) {
+        (void)hipGraphExecDestroy(pl.gexec);
+        pl.gexec = nullptr;
+        pl.g_k = 0;
+    }
     for (int tl = 0; tl < 3; tl++) pl.tp[tl].clear();
     for (int p = 0; p < np; p++) {
         Prob& pr = pl.hp[p];
         const int sx = pr.x1 - pr.xs + 1, sy = pr.y1 - pr.ys + 1, sz = pr.z1 - pr.zs + 1;
+        const bool on = !active || (*active)[p];
         for (int tl = 0; tl < 3; tl++) {
             int nx, ny, nz;
-            if (tl == TL_EL) {
+            if (!on) {
+                nx = ny = nz = 0;
+            } else if (tl == TL_EL) {
                 nx = (sx + 127) / 128; ny = (sy + 3) / 4; nz = sz;
             } else if (pl.three_d) {
                 const Shape3 sh = k3DShapes[pl.shape3];
@@ -610,6 +626,30 @@ cw_status capture_chunk(Plan& pl, int K, int crit, int ce, double tol, int maxs)
     return CW_OK;
 }
 
+// Lays a batch plan out again over its unstopped problems once a quarter of
+// them have stopped, so the tail of a batch does not dispatch workgroups that
+// only find their problem stopped. Called right after a chunk's verdicts were
+// read: the chunk-end decide has already judged the last launch exactly as
+// the next launch would, so that launch is told not to judge it again from
+// partials laid out by the old tiling (prev_check cleared). The change slots
+// are per problem and unaffected.
+cw_status compact_plan(Plan& pl) {
+    if (!pl.compact || pl.nprob < 2) return CW_OK;
+    if (pl.act.empty()) {
+        pl.act.assign(pl.nprob, 1);
+        pl.nact = pl.nprob;
+    }
+    int nrun = 0;
+    for (int p = 0; p < pl.nprob; p++)
+        nrun += pl.hc[p].status == ST_RUNNING || pl.hc[p].status == ST_REPLAY;
+    if (nrun * 4 > pl.nact * 3) return CW_OK;
+    for (int p = 0; p < pl.nprob; p++)
+        pl.act[p] = pl.hc[p].status == ST_RUNNING || pl.hc[p].status == ST_REPLAY;
+    pl.nact = nrun;
+    pl.prev_check = 0;
+    return plan_upload(pl, &pl.act);
+}
+
 // Runs passes until no problem is RUNNING or `budget` sweeps have run; the
 // device stops a problem once the run has done `dev_max` sweeps. Problems
 // that stopped inside a fused pass are replayed one sweep at a time from the
@@ -654,6 +694,7 @@ cw_status run_passes(Plan& pl, int crit, double tol, uint64_t budget, uint64_t d
             if (out) out->graph_replays++;
             if ((st = read_ctl(pl))) return st;
             if (!any_status(pl, ST_RUNNING)) break;
+            if ((st = compact_plan(pl))) return st;
             continue;
         }
         for (int j = 0; j < chunk; j++) {
@@ -668,6 +709,7 @@ cw_status run_passes(Plan& pl, int crit, double tol, uint64_t budget, uint64_t d
         if ((st = read_ctl(pl))) return st;
         if (!any_status(pl, ST_RUNNING)) break;
         if ((uint64_t)pl.sweeps_before - start >= budget) break;
+        if ((st = compact_plan(pl))) return st;
         chunk = std::min(chunk * 2, 64);
     }
     if (any_status(pl, ST_REPLAY)) {
@@ -826,6 +868,7 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
         return fail(CW_ERR_ARG, "unknown volume tile shape %d", o.shape);
     pl.s_main = three_d ? 1 : kPlainShapes[pl.shape].s;
     pl.use_graph = o.use_graph;
+    pl.compact = nf > 1 && o.time_passes == 0;
     if (three_d) {
         // volumes: tile_shape picks a 3D shape, one sweep per launch
         pl.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape
@@ -1080,7 +1123,21 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     const int first = std::max(32, longest / 4);
     const int hi = std::max(256, 4 * longest);
     long passes = 0;
+    // Fields that have settled keep no tiles: once a quarter of the plan's
+    // fields have stopped, both plans are laid out again over the rest, so a
+    // batch's tail does not dispatch workgroups that only find their field
+    // stopped.
+    std::vector<char> act(np, 1);
+    int nact = np;
     for (uint64_t it = 0; it < o.max_outer && any_status(outer, ST_RUNNING); it++) {
+        int nrun = 0;
+        for (int p = 0; p < np; p++) nrun += outer.hc[p].status == ST_RUNNING;
+        if (np > 1 && nrun * 4 <= nact * 3) {
+            for (int p = 0; p < np; p++) act[p] = outer.hc[p].status == ST_RUNNING;
+            if ((st = plan_upload(outer, &act))) return st;
+            if ((st = plan_upload(inner, &act))) return st;
+            nact = nrun;
+        }
         int most = 0;
         for (int p = 0; p < np; p++) {
             const bool run = outer.hc[p].status == ST_RUNNING;
