@@ -153,3 +153,22 @@ def test_a_concurrent_batch_reports_a_field_that_does_not_settle():
     gs = [gpu_from_case(fc) for fc in fcs]
     with pytest.raises(SimulationError, match="did not settle"):
         batch_relax(gs, criterion=CRIT_REL_RESIDUAL, tolerance=1e-12, omega=1.9, max_sweeps=40)
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_a_batch_whose_fields_stop_at_different_times_keeps_their_own_bits(graph):
+    """Eight fields of different sizes to a relative residual: as they stop
+    the batch is laid out again over the rest, and every field still gets
+    the sweeps and bits it gets alone."""
+    say(f"case uneven batch graph={graph}")
+    hist = cases.campaign_histories(seed=4400, count=4, lo=48, hi=200)
+    fcs = [fc for h in hist for fc in h]
+    together = [gpu_from_case(fc) for fc in fcs]
+    st_t, _ = batch_relax(together, criterion=CRIT_REL_RESIDUAL, tolerance=1e-9, omega=-1.0,
+                          use_graph=graph)
+    assert len({s["sweeps"] for s in st_t}) > 2
+    for fc, g, s in zip(fcs, together, st_t):
+        alone = gpu_from_case(fc)
+        s1 = alone.relax_ex(criterion=CRIT_REL_RESIDUAL, tolerance=1e-9, omega=-1.0)
+        assert s["sweeps"] == s1["sweeps"]
+        assert bits_equal(g.download(), alone.download())
