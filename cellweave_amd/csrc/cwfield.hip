@@ -250,6 +250,29 @@ struct Plan {
     hipGraphExec_t gexec = nullptr;
     int g_k = 0, g_crit = -1, g_ce = 0, g_max = 0;
     double g_tol = 0.0;
+    // What a captured graph's launches bake in: buffers, grids, shapes. A
+    // graph is replayed only while the plan still has this layout, so a plan
+    // kept between solves reuses its graph for as long as the field does not
+    // change shape.
+    struct Layout {
+        const void* buf[8];
+        int n[8];
+        bool operator==(const Layout& o) const {
+            return memcmp(this, &o, sizeof *this) == 0;
+        }
+    };
+    Layout layout() const {
+        Layout L;
+        memset(&L, 0, sizeof L);
+        L.buf[0] = dp; L.buf[1] = dc; L.buf[2] = dtp[0]; L.buf[3] = dtp[1];
+        L.buf[4] = dtp[2]; L.buf[5] = partials; L.buf[6] = run;
+        L.n[0] = (int)tp[0].size(); L.n[1] = (int)tp[1].size(); L.n[2] = (int)tp[2].size();
+        L.n[3] = nprob; L.n[4] = part_stride; L.n[5] = s_main;
+        L.n[6] = shape | (shape3 << 8) | (three_d << 16) | (fp64 << 17) | (plain << 18) |
+                 (general << 19);
+        return L;
+    }
+    Layout g_layout{};
     // run_passes lays the plan out again over its unstopped problems
     bool compact = false;
     std::vector<char> act;
@@ -259,8 +282,8 @@ struct Plan {
     // (and dropping a captured graph, whose launch grids and arguments belong
     // to the previous layout).
     void reuse() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        gexec = nullptr;
+        // a captured graph stays: capture_chunk replays it only if the new
+        // solve has the same layout and pattern
         stream = nullptr;
         nprob = 0; s_main = 1; shape = 0;
         fp64 = true; plain = true; general = false; three_d = false;
@@ -271,7 +294,6 @@ struct Plan {
         pass = 0; sweeps_before = 0; prev_s = 0; prev_tl = 0; prev_check = 0;
         use_graph = false;
         run_host[0] = run_host[1] = 0;
-        g_k = 0; g_crit = -1; g_ce = 0; g_max = 0; g_tol = 0.0;
         compact = false; act.clear(); nact = 0;
     }
 
@@ -312,15 +334,10 @@ Plan& field_plan(cw_field* f, int which) {
 // Lays out the tilings of every problem (s-sweep tile, one-sweep tile,
 // elementwise tile) and uploads problems, control blocks and tables.
 // With `active`, problems marked 0 get no tiles: launches skip them without
-// dispatching a workgroup. A captured graph is dropped, since its launch
-// grids follow the tile lists.
+// dispatching a workgroup. (A captured graph whose grids followed the old tile
+// lists no longer matches the plan's layout and is captured again.)
 cw_status plan_upload(Plan& pl, const std::vector<char>* active = nullptr) {
     const int np = pl.nprob;
-    if (pl.gexec) {
-        (void)hipGraphExecDestroy(pl.gexec);
-        pl.gexec = nullptr;
-        pl.g_k = 0;
-    }
     for (int tl = 0; tl < 3; tl++) pl.tp[tl].clear();
     for (int p = 0; p < np; p++) {
         Prob& pr = pl.hp[p];
@@ -569,14 +586,14 @@ struct LoopResult { long passes = 0; long graph_replays = 0; };
 // plan for the same pattern.
 cw_status capture_chunk(Plan& pl, int K, int crit, int ce, double tol, int maxs) {
     const int S = pl.three_d ? 1 : pl.s_main;
+    if (!pl.run) CW_HIP(hipMalloc((void**)&pl.run, 2 * sizeof(int)));
     if (pl.gexec && pl.g_k == K && pl.g_crit == crit && pl.g_ce == ce && pl.g_tol == tol &&
-        pl.g_max == maxs)
+        pl.g_max == maxs && pl.g_layout == pl.layout())
         return CW_OK;
     if (pl.gexec) {
         (void)hipGraphExecDestroy(pl.gexec);
         pl.gexec = nullptr;
     }
-    if (!pl.run) CW_HIP(hipMalloc((void**)&pl.run, 2 * sizeof(int)));
     CW_HIP(hipStreamBeginCapture(pl.stream, hipStreamCaptureModeThreadLocal));
     cw_status st = CW_OK;
     auto pattern_check = [&](int j) { return crit == CRIT_RESID && (j % ce == 0) ? 1 : 0; };
@@ -623,6 +640,7 @@ cw_status capture_chunk(Plan& pl, int K, int crit, int ce, double tol, int maxs)
     if (ei != hipSuccess)
         return fail(CW_ERR_DEVICE, "graph instantiate failed: %s", hipGetErrorString(ei));
     pl.g_k = K; pl.g_crit = crit; pl.g_ce = ce; pl.g_tol = tol; pl.g_max = maxs;
+    pl.g_layout = pl.layout();
     return CW_OK;
 }
 
@@ -1640,6 +1658,10 @@ cw_status cw_field_relax(cw_field* f, double tolerance, uint64_t max_sweeps, uin
     cw_relax_opts_default(&o);
     o.tolerance = tolerance;
     o.max_sweeps = max_sweeps;
+    // Steady chunks replay from a graph the field keeps between calls: the
+    // warm-started relax of every lattice step launches once per 16 sweeps
+    // instead of once per sweep. Same sweeps and bits as eager launches.
+    o.use_graph = 1;
     cw_relax_stats s;
     cw_status st = solve(&f, 1, &o, &s);
     if (sweeps) *sweeps = s.sweeps;
