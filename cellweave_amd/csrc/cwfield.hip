@@ -14,6 +14,8 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -1244,9 +1246,14 @@ cw_status solve_concurrently(cw_field* const* F, uint32_t nf, const Opts& o,
     auto work = [&](uint32_t i) {
         Opts oi = o;
         oi.time_ms = o.time_passes > 0 ? &ms[i] : nullptr;
-        cw_status st = hipSetDevice(dev) == hipSuccess
-                           ? solve_one(F[i], oi, &stats[i])
-                           : fail(CW_ERR_DEVICE, "hipSetDevice(%d) failed in a solve thread", dev);
+        cw_status st;
+        try {
+            st = hipSetDevice(dev) == hipSuccess
+                     ? solve_one(F[i], oi, &stats[i])
+                     : fail(CW_ERR_DEVICE, "hipSetDevice(%d) failed in a solve thread", dev);
+        } catch (const std::exception& e) {
+            st = fail(CW_ERR_DEVICE, "host error in a solve thread: %s", e.what());
+        }
         if (st == CW_OK && hipStreamSynchronize(F[i]->stream) != hipSuccess)
             st = fail(CW_ERR_DEVICE, "stream synchronisation failed in a solve thread");
         sts[i] = st;
@@ -1268,8 +1275,24 @@ cw_status solve_concurrently(cw_field* const* F, uint32_t nf, const Opts& o,
     return CW_OK;
 }
 
+cw_status solve_impl(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts,
+                     cw_relax_stats* stats, long time_passes, double* time_ms);
+
+// Every solve entry point comes through here: no C++ exception crosses the
+// C ABI (host allocations for plans and threads can throw).
 cw_status solve(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts, cw_relax_stats* stats,
                 long time_passes = 0, double* time_ms = nullptr) {
+    try {
+        return solve_impl(F, nf, opts, stats, time_passes, time_ms);
+    } catch (const std::bad_alloc&) {
+        return fail(CW_ERR_DEVICE, "out of host memory");
+    } catch (const std::exception& e) {
+        return fail(CW_ERR_DEVICE, "host error: %s", e.what());
+    }
+}
+
+cw_status solve_impl(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts,
+                     cw_relax_stats* stats, long time_passes, double* time_ms) {
     auto t0 = std::chrono::steady_clock::now();
     if (!F || nf == 0) return fail(CW_ERR_ARG, "no fields");
     Opts o;
