@@ -104,11 +104,13 @@ def solve(fields, plan_, precision=PREC_FP32_REFINE, tol=1e-8, use_graph=1,
     return stats, time.perf_counter() - t0
 
 
-def tune(cases_, per_class=8, launches=10):
-    """Rank the sweep shapes on a sample of each size class, for both precisions."""
+def tune(cases_, per_class=0, launches=10):
+    """Rank the sweep shapes on each size class, for both precisions: on every
+    field of the class (per_class=0), since how many workgroup rounds a launch
+    takes depends on the whole batch, or on its first per_class fields."""
     table = {"fp64": {}, "fp32": {}, "ranking": {}}
     for cls, idx in sorted(group(cases_).items()):
-        sample = [cases_[k] for k in idx[:per_class]]
+        sample = [cases_[k] for k in (idx[:per_class] if per_class > 0 else idx)]
         fields = make_fields(sample)
         for key, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32_REFINE)):
             ranked = autotune(fields, launches=launches, criterion=CRIT_REL_RESIDUAL,
@@ -123,7 +125,7 @@ def tune(cases_, per_class=8, launches=10):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--tune", action="store_true")
-    ap.add_argument("--count", type=int, default=64)
+    ap.add_argument("--count", type=int, default=512)
     args = ap.parse_args()
     from . import cases
 
@@ -131,7 +133,7 @@ def main():
     flat = [fc for h in hist for fc in h]
     if args.tune:
         table = tune(flat)
-        table["workload"] = f"campaign sample: {args.count} histories x 2 fields"
+        table["workload"] = f"campaign: {args.count} plane histories x 2 fields, whole size classes"
         with open(TABLE_PATH, "w") as fh:
             json.dump(table, fh, indent=1)
         print(json.dumps(table))
