@@ -100,6 +100,7 @@ def _declare():
         "cw_lattice_create": (st, [u32, u32, u32, C.POINTER(P)]),
         "cw_lattice_destroy": (st, [P]),
         "cw_lattice_upload": (st, [P, u32p]),
+        "cw_lattice_set": (st, [P, u64p, u32p, u64]),
         "cw_field_apply_uptake": (st, [P, P, u8p, u32, d]),
         "cw_field_apply_uptake_per_cell": (st, [P, P, u8p, dp, u32]),
         "cw_field_apply_background_uptake": (st, [P, P, d]),

@@ -450,6 +450,17 @@ __global__ __launch_bounds__(kBlock) void k_sum_u64(const unsigned long long* sl
     }
 }
 
+// Changed voxels of the lattice: entry j sets voxel v[j] (flat index,
+// x fastest) to id[j]. The host has already kept only the last entry for
+// each voxel, so no two entries write the same word.
+__global__ __launch_bounds__(kBlock) void k_lattice_set(const uint64_t* v, const uint32_t* id,
+                                                        size_t n, int W, int P, uint32_t* ids) {
+    const size_t j = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    const uint64_t row = v[j] / (uint64_t)W;
+    ids[row * (uint64_t)P + (v[j] - row * (uint64_t)W)] = id[j];
+}
+
 // ---- host <-> device copies through a flat staging buffer ---------------------
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_unflatten(const T* flat, T* pitched, int W,
@@ -920,6 +931,45 @@ cw_status cw_lattice_upload(cw_lattice* l, const uint32_t* ids) {
     cw_status st = copy_in(l->stream, (int)l->W, (size_t)l->H * l->D, l->P, l->ids, ids, 4,
                            &l->stage, &l->stage_bytes);
     if (st) return st;
+    l->csr_valid = false;
+    return CW_OK;
+}
+
+cw_status cw_lattice_set(cw_lattice* l, const uint64_t* voxels, const uint32_t* ids,
+                         uint64_t count) {
+    if (!l) return fail(CW_ERR_ARG, "null lattice");
+    if (count == 0) return CW_OK;
+    if (!voxels || !ids) return fail(CW_ERR_ARG, "null voxel or id list");
+    // Keep the last entry for each voxel, as applying the list in order would.
+    std::vector<std::pair<uint64_t, uint64_t>> order((size_t)count);  // (voxel, position)
+    for (uint64_t j = 0; j < count; j++) {
+        if (voxels[j] >= l->n)
+            return fail(CW_ERR_ARG, "voxel %llu outside the %ux%ux%u lattice",
+                        (unsigned long long)voxels[j], l->W, l->H, l->D);
+        order[(size_t)j] = {voxels[j], j};
+    }
+    std::sort(order.begin(), order.end());
+    std::vector<uint64_t> v;
+    std::vector<uint32_t> id;
+    v.reserve((size_t)count);
+    id.reserve((size_t)count);
+    for (size_t k = 0; k < order.size(); k++) {
+        if (k + 1 < order.size() && order[k + 1].first == order[k].first) continue;
+        v.push_back(order[k].first);
+        id.push_back(ids[order[k].second]);
+    }
+    const size_t n = v.size();
+    cw_status st = stage_ensure(&l->stage, &l->stage_bytes, n * (sizeof(uint64_t) + sizeof(uint32_t)));
+    if (st) return st;
+    uint64_t* dv = (uint64_t*)l->stage;
+    uint32_t* did = (uint32_t*)(dv + n);
+    CW_HIP(hipMemcpyAsync(dv, v.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice, l->stream));
+    CW_HIP(hipMemcpyAsync(did, id.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, l->stream));
+    hipLaunchKernelGGL(k_lattice_set, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       l->stream, (const uint64_t*)dv, (const uint32_t*)did, n, (int)l->W, l->P,
+                       l->ids);
+    CW_HIP(hipGetLastError());
+    CW_HIP(hipStreamSynchronize(l->stream));
     l->csr_valid = false;
     return CW_OK;
 }

@@ -95,6 +95,15 @@ class Lattice:
         assert a.size == self.n
         _check(_abi.lib().cw_lattice_upload(self._h, a.ctypes.data_as(C.POINTER(C.c_uint32))))
 
+    def set(self, voxels, ids):
+        """Set the listed voxels (flat indices) to the listed ids, in order:
+        a lattice step's changes without re-uploading the grid."""
+        v = np.ascontiguousarray(voxels, dtype=np.uint64).reshape(-1)
+        i = np.ascontiguousarray(ids, dtype=np.uint32).reshape(-1)
+        assert v.size == i.size
+        _check(_abi.lib().cw_lattice_set(self._h, v.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                         i.ctypes.data_as(C.POINTER(C.c_uint32)), v.size))
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             _abi.lib().cw_lattice_destroy(self._h)

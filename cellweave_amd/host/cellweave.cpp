@@ -49,6 +49,16 @@ void Lattice::set_grid(std::vector<uint32_t> ids) {
     dirty_ = true;
 }
 
+void Lattice::set_ids(const std::vector<uint64_t>& voxels, const std::vector<uint32_t>& ids) {
+    if (voxels.size() != ids.size()) throw std::invalid_argument("voxel and id lists differ");
+    for (size_t j = 0; j < voxels.size(); j++) {
+        if (voxels[j] >= grid_.size()) throw std::invalid_argument("voxel outside the lattice");
+        grid_[voxels[j]] = ids[j];
+    }
+    if (dev_ && !dirty_ && !voxels.empty())
+        check(cw_lattice_set(dev_, voxels.data(), ids.data(), voxels.size()));
+}
+
 void Lattice::set_cells(std::vector<uint8_t> consumes) { consumes_ = std::move(consumes); }
 
 cw_lattice* Lattice::device() {

@@ -69,6 +69,10 @@ class Lattice {
 
     // Replace the ids (width*height*depth, x fastest).
     void set_grid(std::vector<uint32_t> ids);
+    // Set the listed voxels (flat indices) to the listed ids, in order: what
+    // a lattice step changed. A device copy already in use takes only these
+    // voxels (cw_lattice_set) instead of the whole grid.
+    void set_ids(const std::vector<uint64_t>& voxels, const std::vector<uint32_t>& ids);
     // consumes[id] for ids 0..cell_count(); index 0 is the medium.
     void set_cells(std::vector<uint8_t> consumes);
     uint32_t occupant_at(size_t i) const { return grid_.at(i); }

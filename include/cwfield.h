@@ -202,6 +202,13 @@ cw_status cw_batch_time_passes(cw_field *const *fields, uint32_t count,
 cw_status cw_lattice_create(uint32_t width, uint32_t height, uint32_t depth, cw_lattice **out);
 cw_status cw_lattice_destroy(cw_lattice *lattice);
 cw_status cw_lattice_upload(cw_lattice *lattice, const uint32_t *ids);
+/* The voxels a lattice step changed (new; no reference counterpart, it
+ * replaces re-uploading cpm.rs:198's whole grid after a step's accepted
+ * copies): voxel voxels[j] (flat index (z*H + y)*W + x, pde.rs:27-29) takes
+ * id ids[j], in list order, so a voxel listed twice keeps its last id.
+ * CW_ERR_ARG for an index outside the lattice (nothing is changed then). */
+cw_status cw_lattice_set(cw_lattice *lattice, const uint64_t *voxels, const uint32_t *ids,
+                         uint64_t count);
 
 /* coupling.rs:26-39 apply_uptake: clear the sinks, then every voxel of a
  * cell id < ncells with consumes[id] != 0 (coupling.rs:15-20, a living kind)

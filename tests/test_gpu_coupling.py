@@ -334,3 +334,49 @@ def test_device_coupling_then_relax_equals_the_oracle():
     got = g2.relax(1e-12, 200_000)
     assert got == want
     assert bits_equal(g2.download(), ref2.values)
+
+
+def test_changed_voxels_give_the_lattice_a_full_upload_gives():
+    """cw_lattice_set with a step's changed voxels leaves the lattice as a
+    full upload of the new grid does: the same gathers and means, bit for bit."""
+    from cellweave_amd import workloads as wl
+
+    print("case lattice set", flush=True)
+    n = 40
+    seq, ncells = wl.tissue_steps(n=n, radius=15.0, steps=2, copies=200, seed=6300)
+    rates = np.random.default_rng(5).uniform(1e-4, 2e-3, ncells + 1)
+    rates[0] = 0.0
+    by_set, by_upload = Lattice(n, n, n), Lattice(n, n, n)
+    by_set.upload(seq[0])
+    for a, b in zip(seq, seq[1:]):
+        changed = np.flatnonzero(a != b)
+        by_set.set(changed, b[changed])
+    by_upload.upload(seq[-1])
+    f = Field(n, n, n, 20.0, 0.0, 1.0)
+    g = Field(n, n, n, 20.0, 0.0, 1.0)
+    f.apply_uptake_per_cell(by_set, rates)
+    g.apply_uptake_per_cell(by_upload, rates)
+    assert bits_equal(f.state()[1], g.state()[1])
+    f.relax(1e-9, 200_000)
+    assert bits_equal(f.mean_per_cell(by_set, ncells + 1), f.mean_per_cell(by_upload, ncells + 1))
+
+
+def test_a_voxel_listed_twice_keeps_its_last_id():
+    print("case lattice set twice", flush=True)
+    lat = Lattice(8, 6, 1)
+    lat.upload(np.ones(48, np.uint32))
+    lat.set([5, 9, 5], [2, 3, 4])
+    prod = np.zeros(5)
+    prod[4] = 1e-3
+    f = Field(8, 6, 1, 0.2, 0.0, 0.0)
+    f.apply_production(lat, prod)
+    want = np.zeros(48)
+    want[5] = 1e-3
+    assert bits_equal(f.state()[2], want)
+
+
+def test_a_voxel_outside_the_lattice_is_refused():
+    print("case lattice set out of range", flush=True)
+    lat = Lattice(8, 6, 1)
+    with pytest.raises(ValueError, match="outside"):
+        lat.set([48], [1])
