@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -777,6 +778,24 @@ void init_ctl(Ctl& c, int status) {
     c.status = status;
 }
 
+// Starts the timed windows of fields timed concurrently together, so each
+// stream's launches overlap the others' as they do in a solve. A thread that
+// never arrives (an error before its timing) cannot hold the others for
+// long: they wait at most a second.
+struct Gate {
+    std::mutex m;
+    std::condition_variable cv;
+    int expected = 0, arrived = 0;
+    void arrive_and_wait() {
+        std::unique_lock<std::mutex> lk(m);
+        if (++arrived >= expected) {
+            cv.notify_all();
+            return;
+        }
+        cv.wait_for(lk, std::chrono::seconds(1), [&] { return arrived >= expected; });
+    }
+};
+
 struct Opts {
     double omega;
     int crit;
@@ -791,11 +810,16 @@ struct Opts {
     bool use_graph = false;    // replay steady chunks of passes from a captured hipGraph
     long time_passes = 0;      // > 0: time this many sweep launches instead of solving
     double* time_ms = nullptr; // average duration of one sweep launch (HIP events)
+    Gate* time_gate = nullptr; // fields timed concurrently start their windows together
 };
 
 // Times n sweep launches of the plan on its stream with HIP events (the
 // kernels' own durations plus the gaps between back-to-back launches).
-cw_status time_launches(Plan& pl, long n, double* ms) {
+cw_status time_launches(Plan& pl, long n, double* ms, Gate* gate = nullptr) {
+    if (gate) {
+        CW_HIP(hipStreamSynchronize(pl.stream));  // set-up done before the window opens
+        gate->arrive_and_wait();
+    }
     hipEvent_t e0, e1;
     CW_HIP(hipEventCreate(&e0));
     CW_HIP(hipEventCreate(&e1));
@@ -939,7 +963,7 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     if (pl.nprob == 0) return CW_OK;
     if ((st = plan_upload(pl))) return st;
     if (o.time_passes > 0) {
-        if ((st = time_launches(pl, o.time_passes, o.time_ms))) return st;
+        if ((st = time_launches(pl, o.time_passes, o.time_ms, o.time_gate))) return st;
         for (int p = 0; p < pl.nprob; p++)
             if (!pl.in_place() && (pl.pass & 1)) F[idx[p]]->cur ^= 1;
         return CW_OK;
@@ -1126,7 +1150,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     if (o.time_passes > 0) {
         for (int p = 0; p < np; p++) init_ctl(inner.hc[p], ST_RUNNING);
         if ((st = write_ctl(inner))) return st;
-        return time_launches(inner, o.time_passes, o.time_ms);
+        return time_launches(inner, o.time_passes, o.time_ms, o.time_gate);
     }
     std::vector<double> model_rate(np);
     for (int p = 0; p < np; p++) model_rate[p] = model_log_rate(F[idx[p]], outer.hp[p].omega);
@@ -1243,9 +1267,12 @@ cw_status solve_concurrently(cw_field* const* F, uint32_t nf, const Opts& o,
     std::vector<cw_status> sts(nf, CW_OK);
     std::vector<std::string> msgs(nf);
     std::vector<double> ms(nf, 0.0);
+    Gate gate;
+    gate.expected = (int)nf;
     auto work = [&](uint32_t i) {
         Opts oi = o;
         oi.time_ms = o.time_passes > 0 ? &ms[i] : nullptr;
+        oi.time_gate = o.time_passes > 0 ? &gate : nullptr;
         cw_status st;
         try {
             st = hipSetDevice(dev) == hipSuccess
