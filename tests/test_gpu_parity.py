@@ -294,3 +294,39 @@ def test_fp32_refine_3d_in_place_odd_segments(inner_tol):
     say(f"  refine stats {st}")
     assert st["outer_iters"] >= 4
     assert np.max(np.abs(g.download() - ref.values)) < 1e-8
+
+
+@pytest.mark.parametrize("dims", [(90, 70, 1), (30, 26, 22)])
+def test_repeated_relaxes_on_one_field_as_its_free_box_changes(dims):
+    """cw_field_relax keeps its plan and captured graph between calls and
+    captures again when the layout changes. Relax the same field eight times
+    while the held region alternates between patterns with different free
+    boxes (and so different tilings), each time from where the last solve
+    left it, against the oracle doing the same: sweeps and bits every time."""
+    w, h, d = dims
+    say(f"case repeated relaxes {dims}")
+    rng = np.random.default_rng(11 + d)
+    n = w * h * d
+    ref = OracleField(w, h, d, 0.25, 0.01, 0.5)
+    ref.uptake[:] = np.where(rng.random(n) < 0.4, rng.uniform(0, 2e-3, n), 0.0)
+    ref.copy_next_from_values()
+    g = gpu_from_oracle(ref)
+    zz, yy, xx = np.meshgrid(np.arange(d), np.arange(h), np.arange(w), indexing="ij")
+    masks = []
+    for k in range(4):
+        m = (xx < 1 + 3 * k) | (yy >= h - 1 - 2 * k) | (xx == w - 1) | (yy == 0)
+        if d > 1:
+            m |= (zz == 0) | (zz >= d - 1 - k)
+        masks.append(m.reshape(-1).astype(np.uint8))
+    for step in range(8):
+        m = masks[step % 4] if step % 2 == 0 else masks[3 - step % 4]
+        level = 0.4 + 0.1 * step
+        ref.release_all()
+        ref.hold_where(level, m)
+        g.release_all()
+        g.hold_where(m, level)
+        want = ref.relax(1e-9, 200_000)
+        got = g.relax(1e-9, 200_000)
+        say(f"  step {step}: sweeps gpu {got} oracle {want}")
+        assert got == want
+        assert bits_equal(g.download(), ref.values)
