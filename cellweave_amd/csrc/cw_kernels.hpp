@@ -1317,6 +1317,162 @@ __global__ __launch_bounds__(NW * 64) void relax3d_stream(PassArgs a_in) {
     }
 }
 
+// Two red-black sweeps per pass, for the FP32 volume smoother: sweep A runs
+// one plane ahead of sweep B in the same z stream, so each pass over memory
+// advances every voxel twice. At loop step z: A red of plane z, A black of
+// z-1, B red of z-2, B black of z-3, then plane z-3 is stored. Each phase
+// reads only what the sweep order allows: A red(z) the original black cells
+// of z+1 and the not-yet-A-black ones of z-1; A black(z-1) the A-red cells of
+// z-2 (B has not touched them yet) and z; B red(z-2) the A-black cells of
+// z-3 and z-1; B black(z-3) the B-red cells of z-4 and z-2. The halo is four
+// cells wide in x and y (two per sweep) and the A sweep covers the chunk plus
+// three planes either side, so the stored 120 x (NW*NR - 8) x zc block is
+// exact. FP32 smoothing only: no change tracking, no stop inside a pass.
+template <int NW, int NR>
+__global__ __launch_bounds__(NW * 64) void relax3d_stream2(PassArgs a_in) {
+    using T = float;
+    using T2 = float2;
+    const PassArgs a = resolve(a_in);
+    constexpr int RY = NW * NR;
+    constexpr int TX = 120, TY = RY - 8;
+    static_assert(NR >= 2 && TY > 0, "tile shape");
+    __shared__ T edge[2][NW][2][64];  // [slot][wave][first/last row][lane]: one colour
+
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int p = a.tile_prob[t];
+    const Prob& pr = a.probs[p];
+    const int tl = a.tl;
+    const int lt = t - pr.sw_base[tl];
+    const int nxy = pr.ntx[tl] * pr.nty[tl];
+    const int ch = lt / nxy, txy = lt % nxy;
+    const int ti = txy % pr.ntx[tl], tj = txy / pr.ntx[tl];
+    const int gxo = pr.xs + ti * TX - 4;
+    const int gyo = pr.ys + tj * TY - 4;
+    const int za = pr.zs + ch * pr.zc;
+    const int zb = za + pr.zc - 1 < pr.z1 ? za + pr.zc - 1 : pr.z1;
+    const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
+    const size_t plane = (size_t)P * Hh;
+    const int gx = gxo + 2 * lane;
+    const bool colin = gx >= 0 && gx < W;
+    const T* kk = (const T*)pr.k;
+    const T* lbp = (const T*)pr.lb;
+    const bool has_lb = lbp != nullptr;
+    const T d = (T)pr.d, decay = (T)pr.decay, omega = (T)pr.omega, omo = (T)pr.omo;
+    const int r0 = wv * NR;
+    const int rx[2] = {(gx > 0) + (gx < W - 1), (gx + 1 > 0) + (gx + 1 < W - 1)};
+    int ry[NR];
+    bool rowin[NR];
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int gy = gyo + r0 + i;
+        ry[i] = (gy > 0) + (gy < Hh - 1);
+        rowin[i] = colin && gy >= 0 && gy < Hh;
+    }
+    const int base0 = (gxo + gyo + r0) & 1;  // colour of column 2l, row 0, plane 0
+    const T* src = (const T*)pr.x[a.pass & 1];
+
+    auto load_state = [&](Plane<T, NR>& pl, int z) {
+        const int sw0 = (base0 + z) & 1;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            T ve = T(0), vo = T(0);
+            if (rowin[i] && z >= 0 && z < D) {
+                const T2 v = *(const T2*)(src + (size_t)z * plane + (size_t)(gyo + r0 + i) * P + gx);
+                ve = v.x; vo = v.y;
+            }
+            const bool sw = (sw0 ^ (i & 1)) != 0;
+            pl.st[0][i] = sw ? vo : ve;
+            pl.st[1][i] = sw ? ve : vo;
+        }
+    };
+    auto load_coef = [&](PlaneK<T, NR>& K, int z) {
+        const int sw0 = (base0 + z) & 1;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            T ce = nan_of<T>(), co = nan_of<T>(), ue = neg_inf<T>(), uo = ue;
+            if (rowin[i] && z >= 0 && z < D) {
+                const size_t off = (size_t)z * plane + (size_t)(gyo + r0 + i) * P + gx;
+                const T2 c = *(const T2*)(kk + off);
+                ce = c.x; co = c.y;
+                if (has_lb) { const T2 u = *(const T2*)(lbp + off); ue = u.x; uo = u.y; }
+            }
+            const bool sw = (sw0 ^ (i & 1)) != 0;
+            K.k[0][i] = sw ? co : ce; K.k[1][i] = sw ? ce : co;
+            K.u[0][i] = sw ? uo : ue; K.u[1][i] = sw ? ue : uo;
+        }
+    };
+
+    // planes z-4 .. z+1 at loop step z, and the coefficients of z-3 .. z;
+    // the first steps run A red only, so only planes zs0-1, zs0 and the
+    // coefficients of zs0 start loaded
+    Plane<T, NR> q4{}, q3{}, q2{}, q1{}, q0{}, n1{};
+    PlaneK<T, NR> k3{}, k2{}, k1{}, k0{}, kn{};
+    const int zs0 = za - 3;
+    load_state(q1, zs0 - 1);
+    load_state(q0, zs0);
+    load_coef(k0, zs0);
+
+    int par;
+    if (!sweep_prologue(a, pr, p, lt, &par)) return;
+    T* dst = (T*)pr.x[par ^ 1];
+
+    unsigned long long mv = 0ull;
+    int slot = 0;
+    // colour C of plane z: publish the other colour's edge rows, then update
+    auto phase = [&](int C, Plane<T, NR>& cur, const Plane<T, NR>& bz, const Plane<T, NR>& az,
+                     const PlaneK<T, NR>& K, int z) {
+        edge[slot][wv][0][lane] = cur.st[C ^ 1][0];
+        edge[slot][wv][1][lane] = cur.st[C ^ 1][NR - 1];
+        __syncthreads();
+        const T above = wv > 0 ? edge[slot][wv - 1][1][lane] : T(0);
+        const T below = wv < NW - 1 ? edge[slot][wv + 1][0][lane] : T(0);
+        slot ^= 1;
+        const int rz = (z > 0) + (z < D - 1);
+        const int pc = ((base0 + z) & 1) ^ C;  // x parity of colour C on row 0
+        if (C == 0) {
+            if (pc) plane_update<T, false, false, NR, 0, 1>(cur, bz, az, K, above, below, lane, rx, ry, rz, d, decay, omega, omo, T(0), false, r0, 4, 4 + TY, &mv);
+            else plane_update<T, false, false, NR, 0, 0>(cur, bz, az, K, above, below, lane, rx, ry, rz, d, decay, omega, omo, T(0), false, r0, 4, 4 + TY, &mv);
+        } else {
+            if (pc) plane_update<T, false, false, NR, 1, 1>(cur, bz, az, K, above, below, lane, rx, ry, rz, d, decay, omega, omo, T(0), false, r0, 4, 4 + TY, &mv);
+            else plane_update<T, false, false, NR, 1, 0>(cur, bz, az, K, above, below, lane, rx, ry, rz, d, decay, omega, omo, T(0), false, r0, 4, 4 + TY, &mv);
+        }
+    };
+    auto in_box = [&](int z) { return z >= pr.zs && z <= pr.z1; };
+
+    for (int z = zs0; z <= zb + 3; z++) {
+        load_state(n1, z + 1);
+        load_coef(kn, z + 1);
+        if (in_box(z)) phase(0, q0, q1, n1, k0, z);                                 // A red z
+        if (z - 1 >= za - 2 && in_box(z - 1)) phase(1, q1, q2, q0, k1, z - 1);      // A black z-1
+        if (z - 2 >= za - 1 && z - 2 <= zb + 1 && in_box(z - 2))
+            phase(0, q2, q3, q1, k2, z - 2);                                        // B red z-2
+        if (z - 3 >= za && z - 3 <= zb) {
+            phase(1, q3, q4, q2, k3, z - 3);                                        // B black z-3
+            // store plane z-3: interior rows and pairs inside the swept box
+            if (lane >= 2 && lane < 62 && gx <= pr.x1) {
+                const int zz = z - 3;
+                const int sw0 = (base0 + zz) & 1;
+#pragma unroll
+                for (int i = 0; i < NR; i++) {
+                    const int r = r0 + i;
+                    if (r < 4 || r >= 4 + TY) continue;
+                    const int gy = gyo + r;
+                    if (gy > pr.y1) continue;
+                    const bool sw = (sw0 ^ (i & 1)) != 0;
+                    T2 v;
+                    v.x = sw ? q3.st[1][i] : q3.st[0][i];
+                    v.y = sw ? q3.st[0][i] : q3.st[1][i];
+                    *(T2*)(dst + (size_t)zz * plane + (size_t)gy * P + gx) = v;
+                }
+            }
+        }
+        q4 = q3; q3 = q2; q2 = q1; q1 = q0; q0 = n1;
+        k3 = k2; k2 = k1; k1 = k0; k0 = kn;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Elementwise tiling: 128 cells in x (64 pairs) by 4 rows of one plane.
 // ---------------------------------------------------------------------------

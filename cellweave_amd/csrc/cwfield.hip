@@ -102,11 +102,17 @@ constexpr int kNumPlainShapes = (int)(sizeof(kPlainShapes) / sizeof(kPlainShapes
 constexpr int kNumSmoothShapes = (int)(sizeof(kSmoothShapes) / sizeof(kSmoothShapes[0]));
 // 3D shapes: kind 0 = in-place red and black half sweeps (half3d, two
 // launches per sweep); kind 1 = z-streaming relax3d_stream with NW waves of
-// NR rows and zc planes per chunk (one launch per sweep, ping-pong).
+// NR rows and zc planes per chunk (one launch per sweep, ping-pong); kind 2 =
+// the same with two sweeps per launch (below).
 struct Shape3 { int kind, nw, nr, zc; };
 constexpr Shape3 k3DShapes[] = {{0, 4, 1, 1},  {1, 8, 4, 16}, {1, 4, 8, 16}, {1, 8, 4, 32},
                                  {1, 8, 4, 8},  {1, 8, 4, 24}, {1, 4, 8, 32}, {1, 8, 4, 48},
-                                 {1, 4, 8, 24}, {1, 8, 4, 4},  {1, 4, 8, 8},  {1, 4, 8, 4}};
+                                 {1, 4, 8, 24}, {1, 8, 4, 4},  {1, 4, 8, 8},  {1, 4, 8, 4},
+                                 {2, 16, 2, 16}, {2, 16, 2, 32}, {2, 16, 2, 8}};
+// kind 2: two sweeps per launch (relax3d_stream2, FP32 smoothing only), 16
+// waves of 2 rows; its remainder launches use the 8 x 4 one-sweep kernel,
+// which has the same 32-row region
+constexpr bool two_sweep(int shape3) { return k3DShapes[shape3].kind == 2; }
 constexpr int kNum3DShapes = (int)(sizeof(k3DShapes) / sizeof(k3DShapes[0]));
 constexpr int kDefault3DShape = 1;
 
@@ -357,8 +363,11 @@ cw_status plan_upload(Plan& pl, const std::vector<char>* active = nullptr) {
                 if (sh.kind == 0) {
                     nx = ny = nz = 0;
                 } else {
-                    const int ty = sh.nw * sh.nr - 4;
-                    nx = (sx + 123) / 124; ny = (sy + ty - 1) / ty; nz = (sz + sh.zc - 1) / sh.zc;
+                    // two sweeps need a halo of 4 (one sweep 2) in x and y
+                    const bool two = sh.kind == 2 && tl == TL_MAIN;
+                    const int tx = two ? 120 : 124;
+                    const int ty = sh.nw * sh.nr - (two ? 8 : 4);
+                    nx = (sx + tx - 1) / tx; ny = (sy + ty - 1) / ty; nz = (sz + sh.zc - 1) / sh.zc;
                     pr.zc = sh.zc;
                 }
             } else {
@@ -485,11 +494,14 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a) {
 cw_status launch_stream3d(Plan& pl, const PassArgs& a) {
     const int nt = (int)pl.tp[a.tl].size();
     if (nt == 0) return CW_OK;
-    const dim3 grid(nt), b256(256), b512(512);
+    const dim3 grid(nt), b256(256), b512(512), b1024(1024);
     hipStream_t st = pl.stream;
     const bool w4 = k3DShapes[pl.shape3].nw == 4;  // 4 waves of 8 rows, else 8 of 4
 #define CW_L(kern, blk) hipLaunchKernelGGL(kern, grid, blk, 0, st, a)
-    if (pl.fp64 && pl.plain) {
+    if (two_sweep(pl.shape3) && a.tl == TL_MAIN) {
+        if (pl.fp64) return fail(CW_ERR_ARG, "two-sweep volume shapes are for FP32 smoothing only");
+        CW_L((relax3d_stream2<16, 2>), b1024);
+    } else if (pl.fp64 && pl.plain) {
         if (pl.general) {
             if (w4) CW_L((relax3d_stream<double, 4, 8, true, true>), b256);
             else CW_L((relax3d_stream<double, 8, 4, true, true>), b512);
@@ -559,13 +571,16 @@ cw_status one_pass(Plan& pl, int s, int crit, int check, double tol, int max_swe
             a.s = 1;
             if ((st = launch_half3d(pl, a, 0))) return st;
             if ((st = launch_half3d(pl, a, 1))) return st;
+            s = 1;
         } else {
-            PassArgs a = make_args(pl, TL_MAIN, MODE_SWEEP, crit, tol, max_sweeps);
-            a.s = 1;
+            // s sweeps: the main tiling at the plan's depth, the one-sweep
+            // tiling for a remainder
+            const int tl = (s == pl.s_main) ? TL_MAIN : TL_ONE;
+            PassArgs a = make_args(pl, tl, MODE_SWEEP, crit, tol, max_sweeps);
+            a.s = s;
             if ((st = launch_stream3d(pl, a))) return st;
-            pl.prev_tl = TL_MAIN;
+            pl.prev_tl = tl;
         }
-        s = 1;
         check = 0;
     } else {
         const int tl = (s == pl.s_main) ? TL_MAIN : TL_ONE;
@@ -588,7 +603,7 @@ struct LoopResult { long passes = 0; long graph_replays = 0; };
 // verdict and the advance of the device base into one hipGraph; cached per
 // plan for the same pattern.
 cw_status capture_chunk(Plan& pl, int K, int crit, int ce, double tol, int maxs) {
-    const int S = pl.three_d ? 1 : pl.s_main;
+    const int S = pl.s_main;
     if (!pl.run) CW_HIP(hipMalloc((void**)&pl.run, 2 * sizeof(int)));
     if (pl.gexec && pl.g_k == K && pl.g_crit == crit && pl.g_ce == ce && pl.g_tol == tol &&
         pl.g_max == maxs && pl.g_layout == pl.layout())
@@ -681,7 +696,7 @@ cw_status run_passes(Plan& pl, int crit, double tol, uint64_t budget, uint64_t d
     const int maxs = (int)std::min<uint64_t>(dev_max, 0x7fffffff);
     const uint64_t start = (uint64_t)pl.sweeps_before;
     long launched = 0;
-    const int S = pl.three_d ? 1 : pl.s_main;
+    const int S = pl.s_main;
     // Without a stopping rule the run ends at the budget: enqueue all of it
     // and read the verdicts once. With one, start small and double, so a
     // warm start that settles in a few sweeps does not queue many no-op
@@ -827,7 +842,7 @@ cw_status time_launches(Plan& pl, long n, double* ms, Gate* gate = nullptr) {
     run_reset(pl);
     CW_HIP(hipEventRecord(e0, pl.stream));
     for (long i = 0; i < n && st == CW_OK; i++)
-        st = one_pass(pl, pl.three_d ? 1 : pl.s_main, CRIT_NONE, 0, 0.0, 0x7fffffff);
+        st = one_pass(pl, pl.s_main, CRIT_NONE, 0, 0.0, 0x7fffffff);
     CW_HIP(hipEventRecord(e1, pl.stream));
     CW_HIP(hipEventSynchronize(e1));
     float t = 0.0f;
@@ -917,6 +932,9 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
         // volumes: tile_shape picks a 3D shape, one sweep per launch
         pl.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape
                                                                : default_volume_shape(F, nf);
+        if (two_sweep(pl.shape3))
+            return fail(CW_ERR_ARG, "volume shape %d sweeps twice per launch: FP32 smoothing only",
+                        pl.shape3);
         pl.shape = 0;
         pl.s_main = 1;
     }
@@ -1096,9 +1114,12 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     inner.s_main = s;
     inner.shape = shape;
     inner.use_graph = o.use_graph;
-    if (three_d)
+    if (three_d) {
         inner.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape
                                                                  : default_volume_shape(F, nf);
+        inner.s_main = two_sweep(inner.shape3) ? 2 : 1;
+    }
+    const int s_used = inner.s_main;
     std::vector<int> idx;
     std::vector<double> omegas(nf);
     int longest = 2;
@@ -1106,7 +1127,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         cw_field* f = F[i];
         memset(&stats[i], 0, sizeof(cw_relax_stats));
         stats[i].rel_residual = std::nan("");
-        stats[i].sweeps_per_pass = s;
+        stats[i].sweeps_per_pass = s_used;
         if (!f->has_span) continue;
         if ((st = ensure_fp32(f))) return st;
         omegas[i] = pick_omega(o.omega, f);
@@ -1745,6 +1766,7 @@ cw_status cw_autotune(cw_field* const* fields, uint32_t count, const cw_relax_op
     const int n = three_d ? kNum3DShapes : plain ? kNumPlainShapes : kNumSmoothShapes;
     std::vector<std::pair<double, int>> res;
     for (int sh = 0; sh < n; sh++) {
+        if (three_d && plain && two_sweep(sh)) continue;  // FP32 smoothing only
         cw_relax_opts o = *opts;
         o.tile_shape = sh;
         o.sweeps_per_pass = 0;
@@ -1752,7 +1774,7 @@ cw_status cw_autotune(cw_field* const* fields, uint32_t count, const cw_relax_op
         double ms = 0.0;
         cw_status st = solve(fields, count, &o, nullptr, (long)launches, &ms);
         if (st) return st;
-        const int S = three_d ? 1 : plain ? kPlainShapes[sh].s : kSmoothShapes[sh].s;
+        const int S = three_d ? (two_sweep(sh) ? 2 : 1) : plain ? kPlainShapes[sh].s : kSmoothShapes[sh].s;
         res.push_back({1e3 * ms / S, sh});
     }
     std::sort(res.begin(), res.end());

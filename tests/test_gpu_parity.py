@@ -204,6 +204,7 @@ def test_fp32_refine_reaches_residual_oxygen_200(shape):
 
 
 SHAPES_3D = range(12)  # in-place half sweeps (0) and z-streaming variants (1-11)
+SMOOTH_SHAPES_3D = range(15)  # and the two-sweep FP32 smoothers (12-14)
 
 
 def ball_case(side=31, r=10.0, u=0.006, general=False):
@@ -265,7 +266,7 @@ def test_box_3d_rel_residual(shape):
     assert bits_equal(v, ref.values)
 
 
-@pytest.mark.parametrize("shape", SHAPES_3D)
+@pytest.mark.parametrize("shape", SMOOTH_SHAPES_3D)
 def test_fp32_refine_3d(shape):
     say(f"case 3d refine shape={shape}")
     ref, g = ball_case(side=33, r=12.0)
@@ -330,3 +331,29 @@ def test_repeated_relaxes_on_one_field_as_its_free_box_changes(dims):
         say(f"  step {step}: sweeps gpu {got} oracle {want}")
         assert got == want
         assert bits_equal(g.download(), ref.values)
+
+
+def test_two_sweep_volume_shapes_are_refused_for_fp64():
+    say("case two-sweep shape in FP64")
+    ref, _ = ball_case()
+    g = gpu_from_oracle(ref)
+    with pytest.raises(ValueError, match="FP32 smoothing only"):
+        g.relax_ex(omega=1.8, criterion=CRIT_MAX_CHANGE, tolerance=1e-10, tile_shape=12)
+
+
+@pytest.mark.parametrize("two", [12, 13, 14])
+def test_two_sweep_smoother_gives_the_one_sweep_bits(two):
+    """Red-black SOR fixes every cell's operands, whatever order the stream
+    visits planes in: the two-sweep FP32 volume smoother must give the bits
+    of the one-sweep one (same 32-row region, same arithmetic), so the whole
+    refinement matches, sweep for sweep."""
+    say(f"case two-sweep shape {two} against one-sweep")
+    ref, g1 = ball_case(side=37, r=14.0)
+    _, g2 = ball_case(side=37, r=14.0)
+    kw = dict(omega=1.85, criterion=CRIT_REL_RESIDUAL, tolerance=1e-9, max_sweeps=200_000,
+              precision=PREC_FP32_REFINE)
+    s1 = g1.relax_ex(tile_shape=1, **kw)
+    s2 = g2.relax_ex(tile_shape=two, **kw)
+    assert s2["sweeps_per_pass"] == 2
+    assert s1["sweeps"] == s2["sweeps"]
+    assert bits_equal(g1.download(), g2.download())
