@@ -11,7 +11,7 @@ value = voxel-sweeps/s of the FP32+refine solve: sum over fields of (swept
 voxels x smoothing sweeps), divided by the wall time of the step. Inputs are
 resident in HBM: each step resets the device values with cw_field_fill.
 
-Prints one JSON line. --gpus N with N > 1 exits: this build runs on one GPU.
+Prints one JSON line. --gpus N with N > 1 exits with a message: the measurement is of one GPU.
 """
 from __future__ import annotations
 
@@ -162,9 +162,9 @@ def main():
                          "adjoint192, coupling, tissue120")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: every GPU box "
-              "here has one GPU (a batch of fields shards across GPUs with no exchange, a single "
-              "field by slabs with a halo exchange per launch)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (a batch of "
+              "fields would shard across GPUs with no exchange, a single field by slabs with a "
+              "halo exchange per launch)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
