@@ -41,7 +41,7 @@ PLAIN_SHAPES = range(7)  # FP64 plain sweep kernels (LDS tiles and register-bloc
 
 
 @pytest.mark.parametrize("shape", PLAIN_SHAPES)
-def test_config1_max_change_bitwise(shape):
+def test_oxygen_disc_200_max_change_bitwise(shape):
     say(f"case oxygen 200^2 omega=1.9 plain shape={shape}")
     (fc,) = cases.oxygen_disc_200()
     ref = oracle_from_case(fc)
@@ -64,7 +64,7 @@ def test_general_form_every_shape(shape):
     assert bits_equal(g.download(), ref.values)
 
 
-def test_config1_reference_dropin():
+def test_oxygen_disc_200_reference_dropin():
     say("case oxygen 200^2 reference relax (omega rule 1.95)")
     (fc,) = cases.oxygen_disc_200()
     ref = oracle_from_case(fc)
@@ -74,7 +74,7 @@ def test_config1_reference_dropin():
 
 
 @pytest.mark.parametrize("shape", [0, 2, 6])
-def test_config1_rel_residual_bitwise(shape):
+def test_oxygen_disc_200_rel_residual_bitwise(shape):
     say(f"case oxygen 200^2 rel residual 1e-8 one sweep per pass, shape={shape}")
     (fc,) = cases.oxygen_disc_200()
     ref = oracle_from_case(fc)
