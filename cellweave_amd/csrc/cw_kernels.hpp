@@ -868,7 +868,8 @@ __device__ __forceinline__ void reg_half(RegCells<T, NR>& R, T above, T below, i
             // this depth whatever they hold, and never reach a valid cell.
             const T sum = (across + o) + (up + dn);
             after = before + fma_t(R.al[C][i], sum, fma_t(R.ga[C][i], before, R.kc[C][i]));
-            R.st[C][i] = max_t(after, R.k2[C][i]);  // lower bound (-inf without one)
+            // lower bound (GEN: the launch has one; -inf for a field without)
+            R.st[C][i] = GEN ? max_t(after, R.k2[C][i]) : after;
         }
     }
 }
@@ -901,7 +902,9 @@ __global__ __launch_bounds__(NW * 64) void relax2d_reg(PassArgs a_in) {
     const bool colin = gx >= 0 && gx < W;
     const T* kk = (const T*)pr.k;
     const T* k2p = (const T*)(PLAIN ? pr.k2 : pr.lb);
-    const bool has_k2 = PLAIN ? GEN : (k2p != nullptr);
+    // PLAIN: GEN keeps production and uptake apart. Smoother: GEN means the
+    // launch carries lower bounds (a field without one has none to load).
+    const bool has_k2 = PLAIN ? GEN : (GEN && k2p != nullptr);
     const bool replay = a.mode == MODE_REPLAY;
     const T d = (T)pr.d, decay = (T)pr.decay, omega = (T)pr.omega, omo = (T)pr.omo;
     const T source = (T)pr.source;

@@ -239,6 +239,7 @@ struct Plan {
     bool plain = true;   // FP64: floored field (false: the unfloored adjoint)
     bool general = false; // some problem keeps production and uptake apart
     bool three_d = false;
+    bool lower_bound = true;  // FP32 plane smoothing: some problem has a lower bound
     int shape3 = kDefault3DShape;   // k3DShapes index for volumes
     bool in_place() const { return three_d && k3DShapes[shape3].kind == 0; }
     std::vector<Prob> hp;
@@ -278,7 +279,7 @@ struct Plan {
         L.n[0] = (int)tp[0].size(); L.n[1] = (int)tp[1].size(); L.n[2] = (int)tp[2].size();
         L.n[3] = nprob; L.n[4] = part_stride; L.n[5] = s_main;
         L.n[6] = shape | (shape3 << 8) | (three_d << 16) | (fp64 << 17) | (plain << 18) |
-                 (general << 19);
+                 (general << 19) | (lower_bound << 20);
         return L;
     }
     Layout g_layout{};
@@ -295,7 +296,7 @@ struct Plan {
         // solve has the same layout and pattern
         stream = nullptr;
         nprob = 0; s_main = 1; shape = 0;
-        fp64 = true; plain = true; general = false; three_d = false;
+        fp64 = true; plain = true; general = false; three_d = false; lower_bound = true;
         shape3 = kDefault3DShape;
         hp.clear(); hc.clear();
         for (int i = 0; i < 3; i++) tp[i].clear();
@@ -433,6 +434,12 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a) {
     const dim3 grid(nt), b256(256), b512(512), b1024(1024);
     hipStream_t st = pl.stream;
 #define CW_L(kern, blk) hipLaunchKernelGGL(kern, grid, blk, 0, st, a)
+// the register smoother with or without the lower bound of fields with sinks
+#define CW_LB(S_, NW_, NR_, blk)                                                   \
+    do {                                                                           \
+        if (pl.lower_bound) CW_L((relax2d_reg<float, S_, NW_, NR_, false, true>), blk); \
+        else CW_L((relax2d_reg<float, S_, NW_, NR_, false, false>), blk);          \
+    } while (0)
     if (pl.fp64 && pl.plain) {
         if (pl.general) {
             switch (shape) {
@@ -475,17 +482,18 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a) {
         case 9: CW_L((smooth2d_reg<float, 8, 16, 8>), b1024); break;
         case 10: CW_L((smooth2d_reg<float, 2, 8, 8>), b512); break;
         case 11: CW_L((smooth2d_reg<float, 4, 16, 8>), b1024); break;
-        case 12: CW_L((relax2d_reg<float, 2, 4, 8, false, false>), b256); break;
-        case 13: CW_L((relax2d_reg<float, 4, 8, 8, false, false>), b512); break;
-        case 14: CW_L((relax2d_reg<float, 4, 16, 4, false, false>), b1024); break;
-        case 15: CW_L((relax2d_reg<float, 8, 16, 8, false, false>), b1024); break;
-        case 16: CW_L((relax2d_reg<float, 2, 8, 8, false, false>), b512); break;
-        case 17: CW_L((relax2d_reg<float, 4, 16, 8, false, false>), b1024); break;
-        case 18: CW_L((relax2d_reg<float, 6, 16, 8, false, false>), b1024); break;
-        case 19: CW_L((relax2d_reg<float, 10, 16, 8, false, false>), b1024); break;
-        default: CW_L((relax2d_reg<float, 12, 16, 8, false, false>), b1024); break;
+        case 12: CW_LB(2, 4, 8, b256); break;
+        case 13: CW_LB(4, 8, 8, b512); break;
+        case 14: CW_LB(4, 16, 4, b1024); break;
+        case 15: CW_LB(8, 16, 8, b1024); break;
+        case 16: CW_LB(2, 8, 8, b512); break;
+        case 17: CW_LB(4, 16, 8, b1024); break;
+        case 18: CW_LB(6, 16, 8, b1024); break;
+        case 19: CW_LB(10, 16, 8, b1024); break;
+        default: CW_LB(12, 16, 8, b1024); break;
         }
     }
+#undef CW_LB
 #undef CW_L
     CW_HIP(hipGetLastError());
     return CW_OK;
@@ -1120,6 +1128,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         inner.s_main = two_sweep(inner.shape3) ? 2 : 1;
     }
     const int s_used = inner.s_main;
+    inner.lower_bound = false;
     std::vector<int> idx;
     std::vector<double> omegas(nf);
     int longest = 2;
@@ -1153,6 +1162,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         ip.k = f->r32;
         ip.k2 = nullptr;
         ip.lb = pr.lb32;
+        if (ip.lb) inner.lower_bound = true;
         inner.hp.push_back(ip);
         Ctl c;
         init_ctl(c, ST_RUNNING);
