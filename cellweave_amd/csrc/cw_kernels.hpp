@@ -27,6 +27,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 
 namespace cw {
 
@@ -888,7 +889,7 @@ __global__ __launch_bounds__(NW * 64) void relax2d_reg(PassArgs a_in) {
     __shared__ T edge[2][NW][2][2][64];  // [slot][wave][first/last row][colour][lane]
 
     const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int p = a.tile_prob[t];
     const Prob& pr = a.probs[p];
@@ -1035,32 +1036,45 @@ __global__ __launch_bounds__(NW * 64) void relax2d_reg(PassArgs a_in) {
 #pragma unroll
     for (int j = 0; j < (S > kMaxS ? 1 : S); j++) mv[j] = 0ull;
     const bool has_lb = !PLAIN && has_k2;
+    // The half-sweeps. The FP64 sweep is compiled once per colour parity of
+    // row 0 (base_c 0 or 1): each half-sweep is then straight-line code, with
+    // no branch on the region's position and no register copies where two
+    // such branches join. The FP32 smoother keeps one copy that branches on
+    // the parity at run time (base_c -1): two copies of its 24 half-sweeps
+    // measured slower, presumably from the doubled code size.
+    auto sweeps = [&](auto base_c) {
+        constexpr int BC = decltype(base_c)::value;
+        const int B = BC < 0 ? base : BC;
 #pragma unroll
-    for (int h = 0; h < 2 * S; h++) {
-        const int slot = h & 1;
-        const int lo = h + 1, hiY = RY - 2 - h, hiX = RX - 2 - h;
-        // rows above and below hold the other colour at the same column
-        const int oc = (h & 1) ^ 1;
-        const T above = wv > 0 ? edge[slot][wv - 1][1][oc][lane] : T(0);
-        const T below = wv < NW - 1 ? edge[slot][wv + 1][0][oc][lane] : T(0);
-        unsigned long long* m = &mv[PLAIN ? (h >> 1) % (S > kMaxS ? 1 : S) : 0];
-        const int p0 = ((h & 1) ^ base) & 1;  // x parity of this colour on row 0
-        if (h & 1) {
-            if (p0) reg_half<T, PLAIN, GEN, NR, 1, 1, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
-            else reg_half<T, PLAIN, GEN, NR, 1, 0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
-        } else {
-            if (p0) reg_half<T, PLAIN, GEN, NR, 0, 1, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
-            else reg_half<T, PLAIN, GEN, NR, 0, 0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+        for (int h = 0; h < 2 * S; h++) {
+            const int slot = h & 1;
+            const int lo = h + 1, hiY = RY - 2 - h, hiX = RX - 2 - h;
+            // rows above and below hold the other colour at the same column
+            const int oc = (h & 1) ^ 1;
+            const T above = wv > 0 ? edge[slot][wv - 1][1][oc][lane] : T(0);
+            const T below = wv < NW - 1 ? edge[slot][wv + 1][0][oc][lane] : T(0);
+            unsigned long long* m = &mv[PLAIN ? (h >> 1) % (S > kMaxS ? 1 : S) : 0];
+            const int p0 = ((h & 1) ^ B) & 1;  // x parity of this colour on row 0
+            if (h & 1) {
+                if (p0) reg_half<T, PLAIN, GEN, NR, 1, 1, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+                else reg_half<T, PLAIN, GEN, NR, 1, 0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+            } else {
+                if (p0) reg_half<T, PLAIN, GEN, NR, 0, 1, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+                else reg_half<T, PLAIN, GEN, NR, 0, 0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+            }
+            if (h + 1 < 2 * S) {
+                const int ns = slot ^ 1;
+                edge[ns][wv][0][0][lane] = R.st[0][0];
+                edge[ns][wv][0][1][lane] = R.st[1][0];
+                edge[ns][wv][1][0][lane] = R.st[0][NR - 1];
+                edge[ns][wv][1][1][lane] = R.st[1][NR - 1];
+                __syncthreads();
+            }
         }
-        if (h + 1 < 2 * S) {
-            const int ns = slot ^ 1;
-            edge[ns][wv][0][0][lane] = R.st[0][0];
-            edge[ns][wv][0][1][lane] = R.st[1][0];
-            edge[ns][wv][1][0][lane] = R.st[0][NR - 1];
-            edge[ns][wv][1][1][lane] = R.st[1][NR - 1];
-            __syncthreads();
-        }
-    }
+    };
+    if (!PLAIN) sweeps(std::integral_constant<int, -1>{});
+    else if (base) sweeps(std::integral_constant<int, 1>{});
+    else sweeps(std::integral_constant<int, 0>{});
 
     if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= pr.x1) {
 #pragma unroll
