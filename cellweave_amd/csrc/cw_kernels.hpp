@@ -1036,45 +1036,77 @@ __global__ __launch_bounds__(NW * 64) void relax2d_reg(PassArgs a_in) {
 #pragma unroll
     for (int j = 0; j < (S > kMaxS ? 1 : S); j++) mv[j] = 0ull;
     const bool has_lb = !PLAIN && has_k2;
-    // The half-sweeps. The FP64 sweep is compiled once per colour parity of
-    // row 0 (base_c 0 or 1): each half-sweep is then straight-line code, with
-    // no branch on the region's position and no register copies where two
-    // such branches join. The FP32 smoother keeps one copy that branches on
-    // the parity at run time (base_c -1): two copies of its 24 half-sweeps
-    // measured slower, presumably from the doubled code size.
-    auto sweeps = [&](auto base_c) {
-        constexpr int BC = decltype(base_c)::value;
-        const int B = BC < 0 ? base : BC;
+    // The half-sweeps, compiled once per colour parity of row 0 (B), so each
+    // is straight-line code with no branch on the region's position and no
+    // register copies where two such branches would join.
+    if constexpr (PLAIN) {
+        // FP64: unrolled, each sweep's change to its own slot
+        auto sweeps = [&](auto base_c) {
+            constexpr int B = decltype(base_c)::value;
 #pragma unroll
-        for (int h = 0; h < 2 * S; h++) {
-            const int slot = h & 1;
-            const int lo = h + 1, hiY = RY - 2 - h, hiX = RX - 2 - h;
-            // rows above and below hold the other colour at the same column
-            const int oc = (h & 1) ^ 1;
-            const T above = wv > 0 ? edge[slot][wv - 1][1][oc][lane] : T(0);
-            const T below = wv < NW - 1 ? edge[slot][wv + 1][0][oc][lane] : T(0);
-            unsigned long long* m = &mv[PLAIN ? (h >> 1) % (S > kMaxS ? 1 : S) : 0];
-            const int p0 = ((h & 1) ^ B) & 1;  // x parity of this colour on row 0
-            if (h & 1) {
-                if (p0) reg_half<T, PLAIN, GEN, NR, 1, 1, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
-                else reg_half<T, PLAIN, GEN, NR, 1, 0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
-            } else {
-                if (p0) reg_half<T, PLAIN, GEN, NR, 0, 1, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
-                else reg_half<T, PLAIN, GEN, NR, 0, 0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+            for (int h = 0; h < 2 * S; h++) {
+                const int slot = h & 1;
+                const int lo = h + 1, hiY = RY - 2 - h, hiX = RX - 2 - h;
+                // rows above and below hold the other colour at the same column
+                const int oc = (h & 1) ^ 1;
+                const T above = wv > 0 ? edge[slot][wv - 1][1][oc][lane] : T(0);
+                const T below = wv < NW - 1 ? edge[slot][wv + 1][0][oc][lane] : T(0);
+                unsigned long long* m = &mv[(h >> 1) % (S > kMaxS ? 1 : S)];
+                const int p0 = ((h & 1) ^ B) & 1;  // x parity of this colour on row 0
+                if (h & 1) {
+                    if (p0) reg_half<T, PLAIN, GEN, NR, 1, 1, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+                    else reg_half<T, PLAIN, GEN, NR, 1, 0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+                } else {
+                    if (p0) reg_half<T, PLAIN, GEN, NR, 0, 1, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+                    else reg_half<T, PLAIN, GEN, NR, 0, 0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
+                }
+                if (h + 1 < 2 * S) {
+                    const int ns = slot ^ 1;
+                    edge[ns][wv][0][0][lane] = R.st[0][0];
+                    edge[ns][wv][0][1][lane] = R.st[1][0];
+                    edge[ns][wv][1][0][lane] = R.st[0][NR - 1];
+                    edge[ns][wv][1][1][lane] = R.st[1][NR - 1];
+                    __syncthreads();
+                }
             }
-            if (h + 1 < 2 * S) {
-                const int ns = slot ^ 1;
-                edge[ns][wv][0][0][lane] = R.st[0][0];
-                edge[ns][wv][0][1][lane] = R.st[1][0];
-                edge[ns][wv][1][0][lane] = R.st[0][NR - 1];
-                edge[ns][wv][1][1][lane] = R.st[1][NR - 1];
-                __syncthreads();
+        };
+        if (base) sweeps(std::integral_constant<int, 1>{});
+        else sweeps(std::integral_constant<int, 0>{});
+    } else {
+        // FP32 smoother: one sweep (two half-sweeps) per loop iteration. Its
+        // code stays small (two fully unrolled copies of 24 half-sweeps ran
+        // slower), and nothing is scheduled across a barrier anyway.
+        auto sweeps = [&](auto base_c) {
+            constexpr int B = decltype(base_c)::value;
+            auto half = [&](auto colour_c, int h) {
+                constexpr int HC = decltype(colour_c)::value;  // colour updated, h & 1
+                constexpr int P0 = HC ^ B;  // x parity of this colour on row 0
+                const int lo = h + 1, hiY = RY - 2 - h, hiX = RX - 2 - h;
+                // Rows above and below hold the other colour at the same
+                // column. The first and last waves read their own edge
+                // instead: it feeds only rows 0 and RY-1, which are outside
+                // [lo, hiY] at every depth. The unconditional read keeps the
+                // half-sweep one basic block, so the DPP shifts fuse into adds.
+                const T above = edge[HC][wv > 0 ? wv - 1 : wv][1][HC ^ 1][lane];
+                const T below = edge[HC][wv < NW - 1 ? wv + 1 : wv][0][HC ^ 1][lane];
+                reg_half<T, PLAIN, GEN, NR, HC, P0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, &mv[0]);
+                if (h + 1 < 2 * S) {
+                    edge[HC ^ 1][wv][0][0][lane] = R.st[0][0];
+                    edge[HC ^ 1][wv][0][1][lane] = R.st[1][0];
+                    edge[HC ^ 1][wv][1][0][lane] = R.st[0][NR - 1];
+                    edge[HC ^ 1][wv][1][1][lane] = R.st[1][NR - 1];
+                    __syncthreads();
+                }
+            };
+#pragma unroll 1
+            for (int h = 0; h < 2 * S; h += 2) {
+                half(std::integral_constant<int, 0>{}, h);
+                half(std::integral_constant<int, 1>{}, h + 1);
             }
-        }
-    };
-    if (!PLAIN) sweeps(std::integral_constant<int, -1>{});
-    else if (base) sweeps(std::integral_constant<int, 1>{});
-    else sweeps(std::integral_constant<int, 0>{});
+        };
+        if (base) sweeps(std::integral_constant<int, 1>{});
+        else sweeps(std::integral_constant<int, 0>{});
+    }
 
     if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= pr.x1) {
 #pragma unroll

@@ -151,7 +151,7 @@ int default_volume_shape(cw_field* const* F, uint32_t nf) {
 int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 : 15; }
 
 // FP32 planes with no sweep count asked for: each relax2d_reg workgroup
-// (one per CU, 119 VGPRs a lane) loads its tile, about 14 us at full
+// (one per CU, 96-119 VGPRs a lane) loads its tile, about 14 us at full
 // occupancy, then sweeps it S times at about 0.9 us a sweep, and the launch
 // lasts as many of those rounds as it takes to run every tile. A deeper tile
 // spreads the load over more sweeps but leaves a smaller interior, so more
