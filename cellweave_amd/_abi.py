@@ -46,6 +46,8 @@ class RelaxStats(C.Structure):
         ("seconds", C.c_double),
         ("status", C.c_int32),
         ("sweeps_per_pass", C.c_int32),
+        ("tile_shape", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
     def as_dict(self):

@@ -97,7 +97,8 @@ constexpr Shape kSmoothShapes[] = {{1, 32, 4, 0},   {2, 32, 4, 0},   {4, 64, 8, 
                                    {4, 64, 8, 1},   {8, 128, 16, 1}, {2, 64, 8, 1},   {4, 128, 16, 1},
                                    {2, 32, 4, 2},   {4, 64, 8, 2},   {4, 64, 16, 2},  {8, 128, 16, 2},
                                    {2, 64, 8, 2},   {4, 128, 16, 2}, {6, 128, 16, 2},
-                                   {10, 128, 16, 2}, {12, 128, 16, 2}};
+                                   {10, 128, 16, 2}, {12, 128, 16, 2}, {6, 64, 8, 2},
+                                   {8, 64, 8, 2},   {12, 64, 8, 2}};
 constexpr int kNumPlainShapes = (int)(sizeof(kPlainShapes) / sizeof(kPlainShapes[0]));
 constexpr int kNumSmoothShapes = (int)(sizeof(kSmoothShapes) / sizeof(kSmoothShapes[0]));
 // 3D shapes: kind 0 = in-place red and black half sweeps (half3d, two
@@ -156,26 +157,40 @@ int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 
 // lasts as many of those rounds as it takes to run every tile. A deeper tile
 // spreads the load over more sweeps but leaves a smaller interior, so more
 // tiles; pick the S with the least time per sweep over whole rounds.
+//
+// Small batches take 64-row regions at S = 12 (512-thread workgroups) when
+// all their tiles fit in half the CUs: more, shorter workgroups finish a
+// launch sooner than a few 128-row ones. Single planes, FP32+refine to 1e-8
+// (ms, 128-row / 64-row regions): 128^2 0.95 / 0.81, 200^2 4.66 / 4.12,
+// 256^2 1.73 / 1.59, 320^2 2.15 / 2.07, 384^2 2.65 / 2.64 (120 tiles),
+// 448^2 3.02 / 3.21 (168 tiles), 512^2 4.00 / 4.74 (224 tiles).
 constexpr int kPlaneDeepShapes[] = {15, 19, 20};  // S = 8, 10, 12 on 128 x 128 regions
+constexpr int kPlaneShortShape = 23;              // S = 12 on 128 x 64 regions
 constexpr double kTileLoadUs = 14.0, kTileSweepUs = 0.9;
+long plane_tiles(cw_field* const* F, uint32_t nf, int si) {
+    const Shape sh = kSmoothShapes[si];
+    const int tx = 128 - 4 * sh.s, ty = sh.ry - 4 * sh.s;
+    long tiles = 0;
+    for (uint32_t i = 0; i < nf; i++) {
+        const cw_field* f = F[i];
+        if (!f->has_span) continue;
+        const int sx = f->span[1] - f->span[0] + 1, sy = f->span[3] - f->span[2] + 1;
+        tiles += (long)((sx + tx - 1) / tx) * ((sy + ty - 1) / ty);
+    }
+    return tiles;
+}
 int default_smooth2d_shape(cw_field* const* F, uint32_t nf) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         cus <= 0)
         cus = 256;
+    if (plane_tiles(F, nf, kPlaneShortShape) <= cus / 2) return kPlaneShortShape;
     int best = kPlaneDeepShapes[0];
     double best_us = 0.0;
     for (int si : kPlaneDeepShapes) {
         const Shape sh = kSmoothShapes[si];
-        const int tx = 128 - 4 * sh.s, ty = sh.ry - 4 * sh.s;
-        long tiles = 0;
-        for (uint32_t i = 0; i < nf; i++) {
-            const cw_field* f = F[i];
-            if (!f->has_span) continue;
-            const int sx = f->span[1] - f->span[0] + 1, sy = f->span[3] - f->span[2] + 1;
-            tiles += (long)((sx + tx - 1) / tx) * ((sy + ty - 1) / ty);
-        }
+        const long tiles = plane_tiles(F, nf, si);
         const long rounds = std::max(1L, (tiles + cus - 1) / cus);
         const double us = rounds * (kTileLoadUs + sh.s * kTileSweepUs) / sh.s;
         if (si == kPlaneDeepShapes[0] || us < best_us) {
@@ -490,6 +505,9 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a) {
         case 17: CW_LB(4, 16, 8, b1024); break;
         case 18: CW_LB(6, 16, 8, b1024); break;
         case 19: CW_LB(10, 16, 8, b1024); break;
+        case 21: CW_LB(6, 8, 8, b512); break;
+        case 22: CW_LB(8, 8, 8, b512); break;
+        case 23: CW_LB(12, 8, 8, b512); break;
         default: CW_LB(12, 16, 8, b1024); break;
         }
     }
@@ -834,6 +852,10 @@ struct Opts {
     long time_passes = 0;      // > 0: time this many sweep launches instead of solving
     double* time_ms = nullptr; // average duration of one sweep launch (HIP events)
     Gate* time_gate = nullptr; // fields timed concurrently start their windows together
+    // the fields solved concurrently with this one (all of them share the
+    // GPU, so the plane tile shape is chosen for their tiles together)
+    cw_field* const* peers = nullptr;
+    uint32_t npeers = 0;
 };
 
 // Times n sweep launches of the plan on its stream with HIP events (the
@@ -954,6 +976,7 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
         memset(&stats[i], 0, sizeof(cw_relax_stats));
         stats[i].rel_residual = std::nan("");
         stats[i].sweeps_per_pass = s;
+        stats[i].tile_shape = three_d ? pl.shape3 : pl.shape;
         if (!f->has_span) {
             // Nothing to balance: Ok(1) (pde.rs:256-259).
             stats[i].sweeps = o.crit == CRIT_MAXCHANGE ? 1 : 0;
@@ -1104,6 +1127,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     int shape = o.shape >= 0 ? o.shape
                 : o.s      ? default_smooth_shape(o.s)
                 : three_d  ? 0
+                : o.peers  ? default_smooth2d_shape(o.peers, o.npeers)
                            : default_smooth2d_shape(F, nf);
     if (!three_d && shape >= kNumSmoothShapes)
         return fail(CW_ERR_ARG, "unknown tile shape %d", shape);
@@ -1137,6 +1161,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         memset(&stats[i], 0, sizeof(cw_relax_stats));
         stats[i].rel_residual = std::nan("");
         stats[i].sweeps_per_pass = s_used;
+        stats[i].tile_shape = three_d ? inner.shape3 : inner.shape;
         if (!f->has_span) continue;
         if ((st = ensure_fp32(f))) return st;
         omegas[i] = pick_omega(o.omega, f);
@@ -1304,6 +1329,8 @@ cw_status solve_concurrently(cw_field* const* F, uint32_t nf, const Opts& o,
         Opts oi = o;
         oi.time_ms = o.time_passes > 0 ? &ms[i] : nullptr;
         oi.time_gate = o.time_passes > 0 ? &gate : nullptr;
+        oi.peers = F;
+        oi.npeers = nf;
         cw_status st;
         try {
             st = hipSetDevice(dev) == hipSuccess

@@ -87,6 +87,8 @@ typedef struct cw_relax_stats {
     double seconds;           /* wall time of the call */
     int32_t status;           /* cw_status of this field */
     int32_t sweeps_per_pass;  /* s actually used */
+    int32_t tile_shape;       /* sweep tile shape index used (volumes: the volume table) */
+    int32_t reserved;         /* 0 */
 } cw_relax_stats;
 
 /* Fill opts with the reference drop-in: reference omega, max-change

@@ -83,6 +83,7 @@ def test_plane_smoothing_depth_follows_workgroup_rounds():
     st, _ = batch_relax(gs, criterion=CRIT_REL_RESIDUAL, tolerance=1e-3, omega=1.99,
                         precision=PREC_FP32_REFINE)
     assert [s["sweeps_per_pass"] for s in st] == [12, 12, 12]
+    assert [s["tile_shape"] for s in st] == [20, 20, 20]  # 128-row regions
     say("case smoothing depth 2048^2")
     g = Field(2048, 2048, 1, 0.3, 0.05, 0.0)
     g.set_source(2e-3)
@@ -90,6 +91,34 @@ def test_plane_smoothing_depth_follows_workgroup_rounds():
                    precision=PREC_FP32_REFINE, max_sweeps=100_000)
     assert s["sweeps_per_pass"] == 8
     assert np.max(np.abs(g.download() - 2e-3 / 0.05)) <= 1e-5 * (2e-3 / 0.05)
+
+
+def test_small_planes_take_short_regions():
+    """A plane batch whose 64-row, 12-sweep tiles all fit in half the CUs
+    takes them (shape 23); fields solved concurrently count together, so
+    three 256^2 fields (3 x 64 tiles) keep the 128-row regions. Either way
+    the smoother's bits do not depend on the region shape."""
+    say("case short regions 200^2 oxygen")
+    (fc,) = cases.oxygen_disc_200()
+    g = gpu_from_case(fc)
+    s = g.relax_ex(omega=1.9, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8,
+                   precision=PREC_FP32_REFINE)
+    assert s["tile_shape"] == 23 and s["sweeps_per_pass"] == 12
+    v_short = g.download()
+    g.fill(fc.initial)
+    s20 = g.relax_ex(omega=1.9, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8,
+                     precision=PREC_FP32_REFINE, tile_shape=20)
+    assert s20["sweeps"] == s["sweeps"]
+    assert np.array_equal(g.download(), v_short)
+    say("case short regions 3 x 256^2 concurrent")
+    fcs = cases.multifield_plane(n=256)
+    gs = [gpu_from_case(fc) for fc in fcs]
+    st, _ = batch_relax(gs, criterion=CRIT_REL_RESIDUAL, tolerance=1e-3, omega=1.9,
+                        precision=PREC_FP32_REFINE)
+    assert [x["tile_shape"] for x in st] == [20, 20, 20]
+    st1, _ = batch_relax(gs[:1], criterion=CRIT_REL_RESIDUAL, tolerance=1e-3, omega=1.9,
+                         precision=PREC_FP32_REFINE)
+    assert st1[0]["tile_shape"] == 23
 
 
 @pytest.mark.parametrize("precision", [PREC_FP64, PREC_FP32_REFINE])
