@@ -104,21 +104,43 @@ def solve(fields, plan_, precision=PREC_FP32_REFINE, tol=1e-8, use_graph=1,
     return stats, time.perf_counter() - t0
 
 
-def tune(cases_, per_class=0, launches=10):
-    """Rank the sweep shapes on each size class, for both precisions: on every
+def tune(cases_, per_class=0, launches=10, confirm=4, tol=1e-8):
+    """Rank the sweep shapes on each size class, for both precisions, on every
     field of the class (per_class=0), since how many workgroup rounds a launch
-    takes depends on the whole batch, or on its first per_class fields."""
-    table = {"fp64": {}, "fp32": {}, "ranking": {}}
+    takes depends on the whole batch, or on its first per_class fields.
+
+    Two stages: cw_autotune ranks every shape by its launch time per sweep,
+    then the `confirm` best are timed on the class's whole solve, as `solve`
+    runs it (graph replays, per-field omega, refinement segments), and the
+    fastest solve wins. The per-sweep ranking leaves out what a shape changes
+    around its launches: launches per segment, graph chunks, verdicts."""
+    table = {"fp64": {}, "fp32": {}, "ranking": {}, "confirmed_ms": {}}
     for cls, idx in sorted(group(cases_).items()):
         sample = [cases_[k] for k in (idx[:per_class] if per_class > 0 else idx)]
         fields = make_fields(sample)
-        for key, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32_REFINE)):
-            ranked = autotune(fields, launches=launches, criterion=CRIT_REL_RESIDUAL,
-                              tolerance=1e-8, omega=-1.0, precision=prec)
-            table[key][cls] = ranked[0][0]
-            table["ranking"][f"{key} {cls}"] = ranked[:6]
+
+        def reset():
             for g, fc in zip(fields, sample):
                 g.fill(fc.initial)
+
+        for key, prec in (("fp64", PREC_FP64), ("fp32", PREC_FP32_REFINE)):
+            ranked = autotune(fields, launches=launches, criterion=CRIT_REL_RESIDUAL,
+                              tolerance=tol, omega=-1.0, precision=prec)
+            reset()
+            timed = []
+            for shape, _ in ranked[:max(1, confirm)]:
+                best = None
+                for _rep in range(2):  # the first also builds the plan and graph
+                    reset()
+                    _, secs = solve(fields, [(cls, list(range(len(fields))), shape)],
+                                    precision=prec, tol=tol, concurrent=False)
+                    best = secs if best is None else min(best, secs)
+                timed.append([shape, 1e3 * best])
+            timed.sort(key=lambda e: e[1])
+            table[key][cls] = timed[0][0]
+            table["ranking"][f"{key} {cls}"] = ranked[:6]
+            table["confirmed_ms"][f"{key} {cls}"] = timed
+            reset()
     return table
 
 
