@@ -91,7 +91,8 @@ constexpr double kPi = 3.14159265358979323846;
 // one-sweep shape used for remainders and exact-stop replays.
 struct Shape { int s, ry, nw, kind; };
 constexpr Shape kPlainShapes[] = {{1, 32, 4, 0}, {2, 32, 4, 0},  {1, 32, 4, 2}, {2, 32, 4, 2},
-                                  {2, 64, 8, 2}, {2, 64, 16, 2}, {1, 32, 8, 2}};
+                                  {2, 64, 8, 2}, {2, 64, 16, 2}, {1, 32, 8, 2}, {4, 32, 4, 2},
+                                  {4, 64, 8, 2}};
 constexpr Shape kSmoothShapes[] = {{1, 32, 4, 0},   {2, 32, 4, 0},   {4, 64, 8, 0},   {4, 64, 4, 0},
                                    {8, 128, 8, 0},  {8, 128, 16, 0}, {2, 32, 4, 1},   {4, 64, 16, 1},
                                    {4, 64, 8, 1},   {8, 128, 16, 1}, {2, 64, 8, 1},   {4, 128, 16, 1},
@@ -117,7 +118,6 @@ constexpr bool two_sweep(int shape3) { return k3DShapes[shape3].kind == 2; }
 constexpr int kNum3DShapes = (int)(sizeof(k3DShapes) / sizeof(k3DShapes[0]));
 constexpr int kDefault3DShape = 1;
 
-int default_plain_shape(int s) { return s <= 1 ? 2 : 3; }
 
 // Volumes: the z-streaming kernels are latency-bound when workgroups are
 // few, so the halo planes a shorter chunk re-reads matter less than the
@@ -167,8 +167,7 @@ int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 
 constexpr int kPlaneDeepShapes[] = {15, 19, 20};  // S = 8, 10, 12 on 128 x 128 regions
 constexpr int kPlaneShortShape = 23;              // S = 12 on 128 x 64 regions
 constexpr double kTileLoadUs = 14.0, kTileSweepUs = 0.9;
-long plane_tiles(cw_field* const* F, uint32_t nf, int si) {
-    const Shape sh = kSmoothShapes[si];
+long plane_tiles(cw_field* const* F, uint32_t nf, const Shape& sh) {
     const int tx = 128 - 4 * sh.s, ty = sh.ry - 4 * sh.s;
     long tiles = 0;
     for (uint32_t i = 0; i < nf; i++) {
@@ -185,12 +184,12 @@ int default_smooth2d_shape(cw_field* const* F, uint32_t nf) {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         cus <= 0)
         cus = 256;
-    if (plane_tiles(F, nf, kPlaneShortShape) <= cus / 2) return kPlaneShortShape;
+    if (plane_tiles(F, nf, kSmoothShapes[kPlaneShortShape]) <= cus / 2) return kPlaneShortShape;
     int best = kPlaneDeepShapes[0];
     double best_us = 0.0;
     for (int si : kPlaneDeepShapes) {
         const Shape sh = kSmoothShapes[si];
-        const long tiles = plane_tiles(F, nf, si);
+        const long tiles = plane_tiles(F, nf, sh);
         const long rounds = std::max(1L, (tiles + cus - 1) / cus);
         const double us = rounds * (kTileLoadUs + sh.s * kTileSweepUs) / sh.s;
         if (si == kPlaneDeepShapes[0] || us < best_us) {
@@ -199,6 +198,24 @@ int default_smooth2d_shape(cw_field* const* F, uint32_t nf) {
         }
     }
     return best;
+}
+
+// FP64 planes: s sweeps per launch (1, 2 or 4; 4 unless asked otherwise).
+// Four sweeps go on 128 x 32 regions, or on 128 x 64 ones when the batch has
+// more 32-row tiles than CUs. FP64 to 1e-8, ms (2 sweeps on 128 x 32 / 4 on
+// 128 x 32 / 4 on 128 x 64): 200^2 max change 14.1 / 10.1 / 12.4, 256^2
+// 4.79 / 3.43 / 4.29, 512^2 10.3 / 7.09 / 8.92, 1024^2 22.0 / 18.4 / 15.9,
+// three 1024^2 fields 35.9 / 36.0 / 29.9.
+constexpr int kPlainDeep32 = 7, kPlainDeep64 = 8;
+int default_plain_shape(int s, cw_field* const* F, uint32_t nf) {
+    if (s <= 1) return 2;
+    if (s == 2) return 3;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    return plane_tiles(F, nf, kPlainShapes[kPlainDeep32]) <= cus ? kPlainDeep32 : kPlainDeep64;
 }
 
 }  // namespace
@@ -464,6 +481,8 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a) {
             case 3: CW_L((relax2d_reg<double, 2, 4, 8, true, true>), b256); break;
             case 4: CW_L((relax2d_reg<double, 2, 8, 8, true, true>), b512); break;
             case 5: CW_L((relax2d_reg<double, 2, 16, 4, true, true>), b1024); break;
+            case 7: CW_L((relax2d_reg<double, 4, 4, 8, true, true>), b256); break;
+            case 8: CW_L((relax2d_reg<double, 4, 8, 8, true, true>), b512); break;
             default: CW_L((relax2d_reg<double, 1, 8, 4, true, true>), b512); break;
             }
         } else {
@@ -474,6 +493,8 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a) {
             case 3: CW_L((relax2d_reg<double, 2, 4, 8, true, false>), b256); break;
             case 4: CW_L((relax2d_reg<double, 2, 8, 8, true, false>), b512); break;
             case 5: CW_L((relax2d_reg<double, 2, 16, 4, true, false>), b1024); break;
+            case 7: CW_L((relax2d_reg<double, 4, 4, 8, true, false>), b256); break;
+            case 8: CW_L((relax2d_reg<double, 4, 8, 8, true, false>), b512); break;
             default: CW_L((relax2d_reg<double, 1, 8, 4, true, false>), b512); break;
             }
         }
@@ -727,8 +748,10 @@ cw_status run_passes(Plan& pl, int crit, double tol, uint64_t budget, uint64_t d
     // and read the verdicts once. With one, start small and double, so a
     // warm start that settles in a few sweeps does not queue many no-op
     // launches.
+    // A launch runs S sweeps, or one (the one-sweep tiling) for the last
+    // budget % S.
     int chunk = crit == CRIT_NONE
-                    ? (int)std::min<uint64_t>((budget + (uint64_t)S - 1) / (uint64_t)S, 1u << 20)
+                    ? (int)std::min<uint64_t>(budget / (uint64_t)S + budget % (uint64_t)S, 1u << 20)
                     : 4;
     constexpr int kGraphPasses = 16;
     while (true) {
@@ -762,7 +785,7 @@ cw_status run_passes(Plan& pl, int crit, double tol, uint64_t budget, uint64_t d
         for (int j = 0; j < chunk; j++) {
             const uint64_t done = (uint64_t)pl.sweeps_before - start;
             if (done >= budget) break;
-            const int s = (int)std::min<uint64_t>((uint64_t)pl.s_main, budget - done);
+            const int s = budget - done >= (uint64_t)pl.s_main ? pl.s_main : 1;
             const int check = crit == CRIT_RESID && (pl.pass % check_every == 0);
             if ((st = one_pass(pl, s, crit, check, tol, maxs))) return st;
             launched++;
@@ -942,7 +965,7 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     for (uint32_t i = 0; i < nf; i++)
         if ((F[i]->D > 1) != three_d)
             return fail(CW_ERR_ARG, "a batch mixes planes and volumes");
-    int s = o.s ? std::min(o.s, 2) : 2;
+    int s = o.s ? std::min(o.s, kMaxS) : kMaxS;
     if (three_d) s = 1;
     Plan local;
     Plan& pl = nf == 1 ? field_plan(F[0], 0) : local;
@@ -950,7 +973,9 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     pl.fp64 = true;
     pl.three_d = three_d;
     pl.s_main = s;
-    pl.shape = o.shape >= 0 ? o.shape : default_plain_shape(s);
+    pl.shape = o.shape >= 0 ? o.shape
+               : o.peers ? default_plain_shape(s, o.peers, o.npeers)
+                         : default_plain_shape(s, F, nf);
     if (!three_d && pl.shape >= kNumPlainShapes)
         return fail(CW_ERR_ARG, "unknown tile shape %d", pl.shape);
     if (three_d && o.shape >= kNum3DShapes)

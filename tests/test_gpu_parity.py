@@ -37,7 +37,7 @@ def test_bathed_disc_bitwise(size, radius, uptake, tol):
     assert bits_equal(g.download(), ref.values)
 
 
-PLAIN_SHAPES = range(7)  # FP64 plain sweep kernels (LDS tiles and register-blocked)
+PLAIN_SHAPES = range(9)  # FP64 plain sweep kernels (LDS tiles and register-blocked)
 
 
 @pytest.mark.parametrize("shape", PLAIN_SHAPES)
@@ -51,6 +51,34 @@ def test_oxygen_disc_200_max_change_bitwise(shape):
                     tile_shape=shape)
     assert st["sweeps"] == want
     assert bits_equal(g.download(), ref.values)
+
+
+@pytest.mark.parametrize("shape", [3, 7, 8])
+@pytest.mark.parametrize("max_sweeps", [1, 2, 3, 5, 6, 7])
+def test_budget_not_a_multiple_of_the_depth(shape, max_sweeps):
+    """A run whose sweep budget is not a multiple of the launch depth ends
+    with one-sweep launches: exactly max_sweeps sweeps run, as the oracle's."""
+    say(f"case budget {max_sweeps} plain shape={shape}")
+    (fc,) = cases.oxygen_disc_200()
+    ref = oracle_from_case(fc)
+    g = gpu_from_case(fc)
+    with pytest.raises(RelaxError):
+        ref.relax(1e-30, max_sweeps, omega=1.9)
+    st, _ = batch_relax([g], check=False, omega=1.9, criterion=CRIT_MAX_CHANGE, tolerance=1e-30,
+                        max_sweeps=max_sweeps, tile_shape=shape)
+    assert st[0]["status"] != 0 and st[0]["sweeps"] == max_sweeps
+    assert bits_equal(g.download(), ref.values)
+
+
+@pytest.mark.parametrize("shape", [20, 23])
+def test_fp32_budget_not_a_multiple_of_the_depth(shape):
+    """FP32 smoothing capped by max_sweeps mid-segment runs exactly the cap."""
+    say(f"case fp32 budget 37 shape={shape}")
+    (fc,) = cases.oxygen_disc_200()
+    g = gpu_from_case(fc)
+    st, _ = batch_relax([g], check=False, omega=1.9, criterion=CRIT_REL_RESIDUAL, tolerance=1e-30,
+                        max_sweeps=37, precision=PREC_FP32_REFINE, tile_shape=shape)
+    assert st[0]["status"] != 0 and st[0]["sweeps"] == 37
 
 
 @pytest.mark.parametrize("shape", PLAIN_SHAPES)
