@@ -31,7 +31,7 @@
 
 namespace cw {
 
-constexpr int kMaxS = 4;
+constexpr int kMaxS = 8;
 constexpr int kWave = 64;
 
 enum Status : int { ST_RUNNING = 0, ST_DONE = 1, ST_MAXED = 2, ST_REPLAY = 3, ST_IDLE = 4 };

@@ -92,7 +92,7 @@ constexpr double kPi = 3.14159265358979323846;
 struct Shape { int s, ry, nw, kind; };
 constexpr Shape kPlainShapes[] = {{1, 32, 4, 0}, {2, 32, 4, 0},  {1, 32, 4, 2}, {2, 32, 4, 2},
                                   {2, 64, 8, 2}, {2, 64, 16, 2}, {1, 32, 8, 2}, {4, 32, 4, 2},
-                                  {4, 64, 8, 2}};
+                                  {4, 64, 8, 2}, {8, 64, 8, 2}};
 constexpr Shape kSmoothShapes[] = {{1, 32, 4, 0},   {2, 32, 4, 0},   {4, 64, 8, 0},   {4, 64, 4, 0},
                                    {8, 128, 8, 0},  {8, 128, 16, 0}, {2, 32, 4, 1},   {4, 64, 16, 1},
                                    {4, 64, 8, 1},   {8, 128, 16, 1}, {2, 64, 8, 1},   {4, 128, 16, 1},
@@ -200,22 +200,31 @@ int default_smooth2d_shape(cw_field* const* F, uint32_t nf) {
     return best;
 }
 
-// FP64 planes: s sweeps per launch (1, 2 or 4; 4 unless asked otherwise).
-// Four sweeps go on 128 x 32 regions, or on 128 x 64 ones when the batch has
-// more 32-row tiles than CUs. FP64 to 1e-8, ms (2 sweeps on 128 x 32 / 4 on
-// 128 x 32 / 4 on 128 x 64): 200^2 max change 14.1 / 10.1 / 12.4, 256^2
-// 4.79 / 3.43 / 4.29, 512^2 10.3 / 7.09 / 8.92, 1024^2 22.0 / 18.4 / 15.9,
-// three 1024^2 fields 35.9 / 36.0 / 29.9.
-constexpr int kPlainDeep32 = 7, kPlainDeep64 = 8;
+// FP64 planes: s sweeps per launch as asked (1, 2, 4 or 8), else the
+// library's choice: 8 sweeps on 128 x 64 regions (shape 9, one 512-thread
+// workgroup a CU at 136 VGPRs) while the batch's tiles, peers included, fit
+// on the CUs at once, and 4 sweeps on 128 x 64 regions (shape 8, two
+// workgroups a CU) beyond. FP64 to 1e-8, ms (4 sweeps on 128 x 32 / 4 on
+// 128 x 64 / 8 on 128 x 64; tiles of shape 9):
+//   200^2 max change 9.35 / 11.8 / 9.40 (21)    256^2   3.45 / 4.31 / 3.26 (24)
+//   512^2   6.71 / 8.60 / 6.57 (96)             768^2  10.6 / 10.8 / 8.36 (192)
+//   896^2  12.5 / 12.8 / 17.5 (280)             1024^2 17.6 / 15.1 / 20.5 (352)
+//   three 1024^2 fields 35.9 / 29.8 / 48.4 (1056)
+// Two sweeps a launch on 128 x 32 regions took 14.1 ms at 200^2 and 35.9 ms
+// for the three 1024^2 fields.
+constexpr int kPlainDeep32 = 7, kPlainDeep64 = 8, kPlainDeep8 = 9;
 int default_plain_shape(int s, cw_field* const* F, uint32_t nf) {
-    if (s <= 1) return 2;
+    if (s == 1) return 2;
     if (s == 2) return 3;
+    if (s == 8) return kPlainDeep8;
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         cus <= 0)
         cus = 256;
-    return plane_tiles(F, nf, kPlainShapes[kPlainDeep32]) <= cus ? kPlainDeep32 : kPlainDeep64;
+    if (s == 4)
+        return plane_tiles(F, nf, kPlainShapes[kPlainDeep32]) <= cus ? kPlainDeep32 : kPlainDeep64;
+    return plane_tiles(F, nf, kPlainShapes[kPlainDeep8]) <= cus ? kPlainDeep8 : kPlainDeep64;
 }
 
 }  // namespace
@@ -483,6 +492,7 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a) {
             case 5: CW_L((relax2d_reg<double, 2, 16, 4, true, true>), b1024); break;
             case 7: CW_L((relax2d_reg<double, 4, 4, 8, true, true>), b256); break;
             case 8: CW_L((relax2d_reg<double, 4, 8, 8, true, true>), b512); break;
+            case 9: CW_L((relax2d_reg<double, 8, 8, 8, true, true>), b512); break;
             default: CW_L((relax2d_reg<double, 1, 8, 4, true, true>), b512); break;
             }
         } else {
@@ -495,6 +505,7 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a) {
             case 5: CW_L((relax2d_reg<double, 2, 16, 4, true, false>), b1024); break;
             case 7: CW_L((relax2d_reg<double, 4, 4, 8, true, false>), b256); break;
             case 8: CW_L((relax2d_reg<double, 4, 8, 8, true, false>), b512); break;
+            case 9: CW_L((relax2d_reg<double, 8, 8, 8, true, false>), b512); break;
             default: CW_L((relax2d_reg<double, 1, 8, 4, true, false>), b512); break;
             }
         }
@@ -965,7 +976,7 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     for (uint32_t i = 0; i < nf; i++)
         if ((F[i]->D > 1) != three_d)
             return fail(CW_ERR_ARG, "a batch mixes planes and volumes");
-    int s = o.s ? std::min(o.s, kMaxS) : kMaxS;
+    int s = o.s ? std::min(o.s, kMaxS) : 0;  // 0: the library's choice
     if (three_d) s = 1;
     Plan local;
     Plan& pl = nf == 1 ? field_plan(F[0], 0) : local;

@@ -47,22 +47,27 @@ def test_graph_refine_equals_eager():
         assert bits_equal(x.download(), y.download())
 
 
-def test_fp64_planes_sweep_four_per_launch():
-    """FP64 planes take four sweeps a launch by default: 128 x 32 regions
-    (shape 7) while the batch's tiles fit on the CUs, 128 x 64 ones (shape 8)
+def test_fp64_planes_sweep_deep_launches():
+    """FP64 planes take eight sweeps a launch by default (shape 9, 128 x 64
+    regions) while the batch's tiles fit on the CUs at once, four (shape 8)
     beyond. The reference's stopping rule still stops at its own sweep (the
     verdict replays the tail), so the bits match the two-sweep shape's."""
     say("case fp64 depth 200^2 oxygen, max change")
     (fc,) = cases.oxygen_disc_200()
     g = gpu_from_case(fc)
     s = g.relax_ex(omega=1.9, criterion=CRIT_MAX_CHANGE, tolerance=1e-8, max_sweeps=200_000)
-    assert s["tile_shape"] == 7 and s["sweeps_per_pass"] == 4
-    v4 = g.download()
+    assert s["tile_shape"] == 9 and s["sweeps_per_pass"] == 8
+    v8 = g.download()
     g.fill(fc.initial)
     s2 = g.relax_ex(omega=1.9, criterion=CRIT_MAX_CHANGE, tolerance=1e-8, max_sweeps=200_000,
                     tile_shape=3)
     assert s2["sweeps_per_pass"] == 2 and s2["sweeps"] == s["sweeps"]
-    assert bits_equal(g.download(), v4)
+    assert bits_equal(g.download(), v8)
+    g.fill(fc.initial)
+    s4 = g.relax_ex(omega=1.9, criterion=CRIT_MAX_CHANGE, tolerance=1e-8, max_sweeps=200_000,
+                    sweeps_per_pass=4)
+    assert s4["tile_shape"] == 7 and s4["sweeps"] == s["sweeps"]
+    assert bits_equal(g.download(), v8)
     say("case fp64 depth 3 x 1024^2 concurrent")
     fcs = cases.multifield_plane(n=1024)
     gs = [gpu_from_case(f) for f in fcs]

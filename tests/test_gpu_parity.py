@@ -37,7 +37,7 @@ def test_bathed_disc_bitwise(size, radius, uptake, tol):
     assert bits_equal(g.download(), ref.values)
 
 
-PLAIN_SHAPES = range(9)  # FP64 plain sweep kernels (LDS tiles and register-blocked)
+PLAIN_SHAPES = range(10)  # FP64 plain sweep kernels (LDS tiles and register-blocked)
 
 
 @pytest.mark.parametrize("shape", PLAIN_SHAPES)
@@ -53,7 +53,7 @@ def test_oxygen_disc_200_max_change_bitwise(shape):
     assert bits_equal(g.download(), ref.values)
 
 
-@pytest.mark.parametrize("shape", [3, 7, 8])
+@pytest.mark.parametrize("shape", [3, 7, 8, 9])
 @pytest.mark.parametrize("max_sweeps", [1, 2, 3, 5, 6, 7])
 def test_budget_not_a_multiple_of_the_depth(shape, max_sweeps):
     """A run whose sweep budget is not a multiple of the launch depth ends
