@@ -110,7 +110,8 @@ struct Shape3 { int kind, nw, nr, zc; };
 constexpr Shape3 k3DShapes[] = {{0, 4, 1, 1},  {1, 8, 4, 16}, {1, 4, 8, 16}, {1, 8, 4, 32},
                                  {1, 8, 4, 8},  {1, 8, 4, 24}, {1, 4, 8, 32}, {1, 8, 4, 48},
                                  {1, 4, 8, 24}, {1, 8, 4, 4},  {1, 4, 8, 8},  {1, 4, 8, 4},
-                                 {2, 16, 2, 16}, {2, 16, 2, 32}, {2, 16, 2, 8}};
+                                 {2, 16, 2, 16}, {2, 16, 2, 32}, {2, 16, 2, 8}, {1, 8, 4, 2},
+                                 {1, 4, 8, 2}};
 // kind 2: two sweeps per launch (relax3d_stream2, FP32 smoothing only), 16
 // waves of 2 rows; its remainder launches use the 8 x 4 one-sweep kernel,
 // which has the same 32-row region
@@ -121,33 +122,47 @@ constexpr int kDefault3DShape = 1;
 
 // Volumes: the z-streaming kernels are latency-bound when workgroups are
 // few, so the halo planes a shorter chunk re-reads matter less than the
-// workgroups it adds. The longest chunk (16, 8 or 4 planes) that still gives
-// the batch 240 workgroups; failing that, spans under 2^18 voxels take the
-// in-place red and black half sweeps (an elementwise grid, two launches a
-// sweep) and larger ones 4-plane chunks. Measured per sweep on single fields
-// (in-place / 4 / 8 / 16 planes, us):
+// workgroups it adds. Spans under 2^18 voxels take the in-place red and black
+// half sweeps (an elementwise grid, two launches a sweep) unless a chunk
+// below already gives the batch enough workgroups:
+//   FP64: the longest chunk (16, 8 or 4 planes) that gives 240 workgroups;
+//     failing that 4-plane chunks if they give 120, else 2-plane.
+//   FP32: the longest chunk (16, 8, 4 or 2 planes) that gives 256, else
+//     2-plane chunks.
+// Measured per sweep on single fields (in-place / 4 / 8 / 16 planes, us):
 //   FP64  49^3: 9.9 / 19 / 30 / 54    91^3: 90 / 21 / 35 / 62    151^3: 61 / 46 / 39 / 64
 //   FP32  49^3: 10 / 15 / 24 / 42     91^3: 16 / 16 / 25 / 43    151^3: 63 / 27 / 29 / 47
-constexpr int kVolumeInPlace = 0, kVolumeChunk16 = 1, kVolumeChunk8 = 4, kVolumeChunk4 = 9;
-int default_volume_shape(cw_field* const* F, uint32_t nf) {
-    const int chunks[3] = {kVolumeChunk16, kVolumeChunk8, kVolumeChunk4};
-    uint64_t voxels = 0;
-    for (int c = 0; c < 3; c++) {
-        const Shape3 sh = k3DShapes[chunks[c]];
-        const int ty = sh.nw * sh.nr - 4;
-        long tiles = 0;
-        voxels = 0;
-        for (uint32_t i = 0; i < nf; i++) {
-            const cw_field* f = F[i];
-            if (!f->has_span) continue;
-            const int sx = f->span[1] - f->span[0] + 1, sy = f->span[3] - f->span[2] + 1;
-            const int sz = f->span[5] - f->span[4] + 1;
-            tiles += (long)((sx + 123) / 124) * ((sy + ty - 1) / ty) * ((sz + sh.zc - 1) / sh.zc);
-            voxels += (uint64_t)sx * sy * sz;
-        }
-        if (tiles >= 240) return chunks[c];
+// and whole solves of spheroids to 1e-8 (ms; 2 / 4 / 8 planes, tiles at 4):
+//   FP64  96^3: 6.05 / 7.62 / 10.7 (96)   120^3: 11.8 / 10.7 / 15.0 (150)   160^3: 28.8 / 26.0 / 23.1 (480)
+//   FP32  96^3: 3.33 / 4.48 / 6.82        120^3: 5.77 / 6.06 / 9.19         160^3: 14.1 / 11.8 / 13.6
+constexpr int kVolumeInPlace = 0, kVolumeChunk16 = 1, kVolumeChunk8 = 4, kVolumeChunk4 = 9,
+              kVolumeChunk2 = 15;
+long volume_tiles(cw_field* const* F, uint32_t nf, int si, uint64_t* voxels) {
+    const Shape3 sh = k3DShapes[si];
+    const int ty = sh.nw * sh.nr - 4;
+    long tiles = 0;
+    *voxels = 0;
+    for (uint32_t i = 0; i < nf; i++) {
+        const cw_field* f = F[i];
+        if (!f->has_span) continue;
+        const int sx = f->span[1] - f->span[0] + 1, sy = f->span[3] - f->span[2] + 1;
+        const int sz = f->span[5] - f->span[4] + 1;
+        tiles += (long)((sx + 123) / 124) * ((sy + ty - 1) / ty) * ((sz + sh.zc - 1) / sh.zc);
+        *voxels += (uint64_t)sx * sy * sz;
     }
-    return voxels < (1u << 18) ? kVolumeInPlace : kVolumeChunk4;
+    return tiles;
+}
+int default_volume_shape(cw_field* const* F, uint32_t nf, bool fp64) {
+    uint64_t voxels = 0;
+    if (fp64) {
+        for (int c : {kVolumeChunk16, kVolumeChunk8, kVolumeChunk4})
+            if (volume_tiles(F, nf, c, &voxels) >= 240) return c;
+        if (voxels < (1u << 18)) return kVolumeInPlace;
+        return volume_tiles(F, nf, kVolumeChunk4, &voxels) >= 120 ? kVolumeChunk4 : kVolumeChunk2;
+    }
+    for (int c : {kVolumeChunk16, kVolumeChunk8, kVolumeChunk4, kVolumeChunk2})
+        if (volume_tiles(F, nf, c, &voxels) >= 256) return c;
+    return voxels < (1u << 18) ? kVolumeInPlace : kVolumeChunk2;
 }
 int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 : 15; }
 
@@ -997,7 +1012,7 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     if (three_d) {
         // volumes: tile_shape picks a 3D shape, one sweep per launch
         pl.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape
-                                                               : default_volume_shape(F, nf);
+                                                               : default_volume_shape(F, nf, true);
         if (two_sweep(pl.shape3))
             return fail(CW_ERR_ARG, "volume shape %d sweeps twice per launch: FP32 smoothing only",
                         pl.shape3);
@@ -1184,7 +1199,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     inner.use_graph = o.use_graph;
     if (three_d) {
         inner.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape
-                                                                 : default_volume_shape(F, nf);
+                                                                 : default_volume_shape(F, nf, false);
         inner.s_main = two_sweep(inner.shape3) ? 2 : 1;
     }
     const int s_used = inner.s_main;

@@ -76,6 +76,22 @@ def test_fp64_planes_sweep_deep_launches():
     assert [x["sweeps_per_pass"] for x in st] == [4, 4, 4]
 
 
+@pytest.mark.parametrize("n,fp64_shape,fp32_shape", [(96, 15, 15), (120, 9, 15), (160, 4, 9)])
+def test_volume_chunk_follows_workgroups(n, fp64_shape, fp32_shape):
+    """Volumes take the longest z chunk that still gives the batch enough
+    workgroups: 240 in FP64 (then 4-plane chunks down to 120, else 2-plane),
+    256 in FP32 (down to 2-plane chunks)."""
+    say(f"case volume chunk {n}^3")
+    (fc,) = cases.spheroid(n=n, ncells=int(round(60000 * (n / 256) ** 3)), radius=110 * n / 256,
+                           necrotic=40 * n / 256)
+    g = gpu_from_case(fc)
+    for prec, want in ((PREC_FP64, fp64_shape), (PREC_FP32_REFINE, fp32_shape)):
+        g.fill(fc.initial)
+        st, _ = batch_relax([g], criterion=CRIT_REL_RESIDUAL, tolerance=1e-3, omega=-1.0,
+                            precision=prec)
+        assert st[0]["tile_shape"] == want and st[0]["status"] == 0
+
+
 @pytest.mark.parametrize("precision", [PREC_FP64, PREC_FP32_REFINE])
 def test_small_campaign_converges_and_matches_single_solves(precision):
     say(f"case small campaign precision={precision}")
