@@ -111,7 +111,7 @@ constexpr Shape3 k3DShapes[] = {{0, 4, 1, 1},  {1, 8, 4, 16}, {1, 4, 8, 16}, {1,
                                  {1, 8, 4, 8},  {1, 8, 4, 24}, {1, 4, 8, 32}, {1, 8, 4, 48},
                                  {1, 4, 8, 24}, {1, 8, 4, 4},  {1, 4, 8, 8},  {1, 4, 8, 4},
                                  {2, 16, 2, 16}, {2, 16, 2, 32}, {2, 16, 2, 8}, {1, 8, 4, 2},
-                                 {1, 4, 8, 2}};
+                                 {1, 4, 8, 2},  {1, 8, 4, 1}};
 // kind 2: two sweeps per launch (relax3d_stream2, FP32 smoothing only), 16
 // waves of 2 rows; its remainder launches use the 8 x 4 one-sweep kernel,
 // which has the same 32-row region
@@ -122,21 +122,30 @@ constexpr int kDefault3DShape = 1;
 
 // Volumes: the z-streaming kernels are latency-bound when workgroups are
 // few, so the halo planes a shorter chunk re-reads matter less than the
-// workgroups it adds. Spans under 2^18 voxels take the in-place red and black
-// half sweeps (an elementwise grid, two launches a sweep) unless a chunk
-// below already gives the batch enough workgroups:
-//   FP64: the longest chunk (16, 8 or 4 planes) that gives 240 workgroups;
-//     failing that 4-plane chunks if they give 120, else 2-plane.
-//   FP32: the longest chunk (16, 8, 4 or 2 planes) that gives 256, else
-//     2-plane chunks.
-// Measured per sweep on single fields (in-place / 4 / 8 / 16 planes, us):
-//   FP64  49^3: 9.9 / 19 / 30 / 54    91^3: 90 / 21 / 35 / 62    151^3: 61 / 46 / 39 / 64
-//   FP32  49^3: 10 / 15 / 24 / 42     91^3: 16 / 16 / 25 / 43    151^3: 63 / 27 / 29 / 47
-// and whole solves of spheroids to 1e-8 (ms; 2 / 4 / 8 planes, tiles at 4):
-//   FP64  96^3: 6.05 / 7.62 / 10.7 (96)   120^3: 11.8 / 10.7 / 15.0 (150)   160^3: 28.8 / 26.0 / 23.1 (480)
-//   FP32  96^3: 3.33 / 4.48 / 6.82        120^3: 5.77 / 6.06 / 9.19         160^3: 14.1 / 11.8 / 13.6
+// workgroups it adds, down to a point:
+//   FP64: the longest chunk of 16 or 8 planes that gives the batch 220
+//     workgroups, else the longest of 4, 2 or 1 plane that gives 150;
+//   FP32: the longest chunk of 16, 8, 4 or 2 planes that gives 256, else 2
+//     planes if they give 150, else 1 plane if it gives 150;
+// failing both, spans under 2^18 voxels take the in-place red and black half
+// sweeps (an elementwise grid, two launches a sweep) and larger ones 1-plane
+// chunks. Whole solves of spheroids to 1e-8, ms (in-place / 1 / 2 / 4 / 8
+// planes; the tiles of the chosen chunk):
+//   FP64  64^3: 3.61 / 3.30 / 3.85 / 4.84 / -     (186)
+//         80^3: 5.10 / 4.31 / 4.97 / 6.26 / -     (234)
+//         96^3: 7.39 / 7.27 / 6.15 / 7.73 / 10.7  (188)
+//        120^3: 11.2 / 12.1 / 11.8 / 10.8 / 15.0  (150)
+//        160^3: 36.7 / -    / 28.8 / 26.0 / 23.1  (240)
+//   FP32  64^3: 2.23 / 1.88 / 2.24 / 2.96 / -     (186)
+//         80^3: 3.14 / 2.28 / 2.76 / 3.74 / -     (234)
+//         96^3: 4.92 / 3.56 / 3.39 / 4.55 / 6.82  (188)
+//        120^3: 8.29 / 6.17 / 5.81 / 6.10 / 9.19  (295)
+//        160^3: 33.3 / -    / 14.1 / 11.8 / 13.6  (480)
+// and per sweep on tissue fields (in-place / 4 / 8 / 16 planes, us): FP64
+// 151^3 61 / 46 / 39 / 64, FP32 151^3 63 / 27 / 29 / 47; 16 planes are best
+// at 254^3 (480 workgroups).
 constexpr int kVolumeInPlace = 0, kVolumeChunk16 = 1, kVolumeChunk8 = 4, kVolumeChunk4 = 9,
-              kVolumeChunk2 = 15;
+              kVolumeChunk2 = 15, kVolumeChunk1 = 17;
 long volume_tiles(cw_field* const* F, uint32_t nf, int si, uint64_t* voxels) {
     const Shape3 sh = k3DShapes[si];
     const int ty = sh.nw * sh.nr - 4;
@@ -155,14 +164,17 @@ long volume_tiles(cw_field* const* F, uint32_t nf, int si, uint64_t* voxels) {
 int default_volume_shape(cw_field* const* F, uint32_t nf, bool fp64) {
     uint64_t voxels = 0;
     if (fp64) {
-        for (int c : {kVolumeChunk16, kVolumeChunk8, kVolumeChunk4})
-            if (volume_tiles(F, nf, c, &voxels) >= 240) return c;
-        if (voxels < (1u << 18)) return kVolumeInPlace;
-        return volume_tiles(F, nf, kVolumeChunk4, &voxels) >= 120 ? kVolumeChunk4 : kVolumeChunk2;
+        for (int c : {kVolumeChunk16, kVolumeChunk8})
+            if (volume_tiles(F, nf, c, &voxels) >= 220) return c;
+        for (int c : {kVolumeChunk4, kVolumeChunk2, kVolumeChunk1})
+            if (volume_tiles(F, nf, c, &voxels) >= 150) return c;
+    } else {
+        for (int c : {kVolumeChunk16, kVolumeChunk8, kVolumeChunk4, kVolumeChunk2})
+            if (volume_tiles(F, nf, c, &voxels) >= 256) return c;
+        for (int c : {kVolumeChunk2, kVolumeChunk1})
+            if (volume_tiles(F, nf, c, &voxels) >= 150) return c;
     }
-    for (int c : {kVolumeChunk16, kVolumeChunk8, kVolumeChunk4, kVolumeChunk2})
-        if (volume_tiles(F, nf, c, &voxels) >= 256) return c;
-    return voxels < (1u << 18) ? kVolumeInPlace : kVolumeChunk2;
+    return voxels < (1u << 18) ? kVolumeInPlace : kVolumeChunk1;
 }
 int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 : 15; }
 

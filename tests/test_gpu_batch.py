@@ -76,11 +76,12 @@ def test_fp64_planes_sweep_deep_launches():
     assert [x["sweeps_per_pass"] for x in st] == [4, 4, 4]
 
 
-@pytest.mark.parametrize("n,fp64_shape,fp32_shape", [(96, 15, 15), (120, 9, 15), (160, 4, 9)])
+@pytest.mark.parametrize("n,fp64_shape,fp32_shape",
+                         [(64, 17, 17), (80, 17, 17), (96, 15, 15), (120, 9, 15), (160, 4, 9)])
 def test_volume_chunk_follows_workgroups(n, fp64_shape, fp32_shape):
     """Volumes take the longest z chunk that still gives the batch enough
-    workgroups: 240 in FP64 (then 4-plane chunks down to 120, else 2-plane),
-    256 in FP32 (down to 2-plane chunks)."""
+    workgroups: FP64 16 or 8 planes at 220, else 4, 2 or 1 plane at 150;
+    FP32 16 to 2 planes at 256, else 2 or 1 plane at 150."""
     say(f"case volume chunk {n}^3")
     (fc,) = cases.spheroid(n=n, ncells=int(round(60000 * (n / 256) ** 3)), radius=110 * n / 256,
                            necrotic=40 * n / 256)

@@ -231,8 +231,8 @@ def test_fp32_refine_reaches_residual_oxygen_200(shape):
     assert st["outer_iters"] >= 1
 
 
-SHAPES_3D = [*range(12), 15, 16]  # in-place half sweeps (0) and z-streaming variants
-SMOOTH_SHAPES_3D = range(17)  # and the two-sweep FP32 smoothers (12-14)
+SHAPES_3D = [*range(12), 15, 16, 17]  # in-place half sweeps (0) and z-streaming variants
+SMOOTH_SHAPES_3D = range(18)  # and the two-sweep FP32 smoothers (12-14)
 
 
 def ball_case(side=31, r=10.0, u=0.006, general=False):
