@@ -99,7 +99,7 @@ constexpr Shape kSmoothShapes[] = {{1, 32, 4, 0},   {2, 32, 4, 0},   {4, 64, 8, 
                                    {2, 32, 4, 2},   {4, 64, 8, 2},   {4, 64, 16, 2},  {8, 128, 16, 2},
                                    {2, 64, 8, 2},   {4, 128, 16, 2}, {6, 128, 16, 2},
                                    {10, 128, 16, 2}, {12, 128, 16, 2}, {6, 64, 8, 2},
-                                   {8, 64, 8, 2},   {12, 64, 8, 2}};
+                                   {8, 64, 8, 2},   {12, 64, 8, 2},  {16, 128, 16, 2}};
 constexpr int kNumPlainShapes = (int)(sizeof(kPlainShapes) / sizeof(kPlainShapes[0]));
 constexpr int kNumSmoothShapes = (int)(sizeof(kSmoothShapes) / sizeof(kSmoothShapes[0]));
 // 3D shapes: kind 0 = in-place red and black half sweeps (half3d, two
@@ -183,16 +183,19 @@ int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 
 // occupancy, then sweeps it S times at about 0.9 us a sweep, and the launch
 // lasts as many of those rounds as it takes to run every tile. A deeper tile
 // spreads the load over more sweeps but leaves a smaller interior, so more
-// tiles; pick the S with the least time per sweep over whole rounds.
+// tiles; pick the S (8, 10, 12 or 16) with the least time per sweep over
+// whole rounds.
 //
-// Small batches take 64-row regions at S = 12 (512-thread workgroups) when
-// all their tiles fit in half the CUs: more, shorter workgroups finish a
-// launch sooner than a few 128-row ones. Single planes, FP32+refine to 1e-8
-// (ms, 128-row / 64-row regions): 128^2 0.95 / 0.81, 200^2 4.66 / 4.12,
-// 256^2 1.73 / 1.59, 320^2 2.15 / 2.07, 384^2 2.65 / 2.64 (120 tiles),
-// 448^2 3.02 / 3.21 (168 tiles), 512^2 4.00 / 4.74 (224 tiles).
-constexpr int kPlaneDeepShapes[] = {15, 19, 20};  // S = 8, 10, 12 on 128 x 128 regions
-constexpr int kPlaneShortShape = 23;              // S = 12 on 128 x 64 regions
+// Very small batches take 128 x 64 regions at S = 12 (512-thread workgroups)
+// when all their tiles fit in a fifth of the CUs: more, shorter workgroups
+// finish a launch sooner than a few 128-row ones. FP32+refine to 1e-8, ms
+// (128 x 64 at S = 12 / 128 x 128 at S = 12 / 128 x 128 at S = 16):
+//   128^2 0.87 / 0.98 / 0.89    200^2 4.18 / 4.76 / 4.26    256^2 1.67 / 1.78 / 1.61
+//   320^2 2.07 / 2.16 / 1.96    384^2 2.64 / 2.62 / 2.44    512^2 - / 3.99 / 3.59
+//   1024^2 - / 7.28 / 6.73; three 256^2 fields 2.36 / 2.19 / 1.98; two 512^2
+//   5.36 / 4.16 / 3.66; three 1024^2 - / 12.2 / 14.2 (768 tiles at S = 16)
+constexpr int kPlaneDeepShapes[] = {15, 19, 20, 24};  // S = 8, 10, 12, 16 on 128 x 128 regions
+constexpr int kPlaneShortShape = 23;                  // S = 12 on 128 x 64 regions
 constexpr double kTileLoadUs = 14.0, kTileSweepUs = 0.9;
 long plane_tiles(cw_field* const* F, uint32_t nf, const Shape& sh) {
     const int tx = 128 - 4 * sh.s, ty = sh.ry - 4 * sh.s;
@@ -211,7 +214,7 @@ int default_smooth2d_shape(cw_field* const* F, uint32_t nf) {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         cus <= 0)
         cus = 256;
-    if (plane_tiles(F, nf, kSmoothShapes[kPlaneShortShape]) <= cus / 2) return kPlaneShortShape;
+    if (plane_tiles(F, nf, kSmoothShapes[kPlaneShortShape]) <= cus / 5) return kPlaneShortShape;
     int best = kPlaneDeepShapes[0];
     double best_us = 0.0;
     for (int si : kPlaneDeepShapes) {
@@ -567,6 +570,7 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a) {
         case 21: CW_LB(6, 8, 8, b512); break;
         case 22: CW_LB(8, 8, 8, b512); break;
         case 23: CW_LB(12, 8, 8, b512); break;
+        case 24: CW_LB(16, 16, 8, b1024); break;
         default: CW_LB(12, 16, 8, b1024); break;
         }
     }

@@ -140,10 +140,11 @@ def test_plane_smoothing_depth_follows_workgroup_rounds():
 
 
 def test_small_planes_take_short_regions():
-    """A plane batch whose 64-row, 12-sweep tiles all fit in half the CUs
-    takes them (shape 23); fields solved concurrently count together, so
-    three 256^2 fields (3 x 64 tiles) keep the 128-row regions. Either way
-    the smoother's bits do not depend on the region shape."""
+    """A plane batch whose 64-row, 12-sweep tiles all fit in a fifth of the
+    CUs takes them (shape 23); larger small batches take 16 sweeps on 128-row
+    regions (shape 24) while those fit in one round. Fields solved
+    concurrently count together. The smoother's bits do not depend on the
+    region shape."""
     say("case short regions 200^2 oxygen")
     (fc,) = cases.oxygen_disc_200()
     g = gpu_from_case(fc)
@@ -156,21 +157,23 @@ def test_small_planes_take_short_regions():
                      precision=PREC_FP32_REFINE, tile_shape=20)
     assert s20["sweeps"] == s["sweeps"]
     assert np.array_equal(g.download(), v_short)
-    say("case short regions 3 x 256^2 concurrent")
+    say("case 16 sweeps 3 x 256^2 concurrent")
     fcs = cases.multifield_plane(n=256)
     gs = [gpu_from_case(fc) for fc in fcs]
     st, _ = batch_relax(gs, criterion=CRIT_REL_RESIDUAL, tolerance=1e-3, omega=1.9,
                         precision=PREC_FP32_REFINE)
-    assert [x["tile_shape"] for x in st] == [20, 20, 20]
+    assert [x["tile_shape"] for x in st] == [24, 24, 24]
+    assert [x["sweeps_per_pass"] for x in st] == [16, 16, 16]
     st1, _ = batch_relax(gs[:1], criterion=CRIT_REL_RESIDUAL, tolerance=1e-3, omega=1.9,
                          precision=PREC_FP32_REFINE)
-    assert st1[0]["tile_shape"] == 23
+    assert st1[0]["tile_shape"] == 24
 
 
 @pytest.mark.parametrize("precision", [PREC_FP64, PREC_FP32_REFINE])
 def test_a_few_fields_solve_concurrently_with_their_own_bits(precision):
     """Two to four fields run as concurrent per-field solves on their own
-    streams: each gets the sweeps and bits it gets when solved alone."""
+    streams: each gets the sweeps and bits it gets when solved alone with the
+    tile shape the batch chose (the choice counts the whole batch's tiles)."""
     say(f"case concurrent batch precision={precision}")
     fcs = cases.multifield_plane(n=160)
     together = [gpu_from_case(fc) for fc in fcs]
@@ -179,7 +182,7 @@ def test_a_few_fields_solve_concurrently_with_their_own_bits(precision):
     for fc, g, s in zip(fcs, together, st_t):
         alone = gpu_from_case(fc)
         (s1,), _ = batch_relax([alone], criterion=CRIT_REL_RESIDUAL, tolerance=1e-9, omega=1.95,
-                               precision=precision)
+                               precision=precision, tile_shape=s["tile_shape"])
         assert s["status"] == 0 and s1["status"] == 0
         assert s["sweeps"] == s1["sweeps"]
         assert bits_equal(g.download(), alone.download())
@@ -244,6 +247,7 @@ def test_a_batch_whose_fields_stop_at_different_times_keeps_their_own_bits(graph
     assert len({s["sweeps"] for s in st_t}) > 2
     for fc, g, s in zip(fcs, together, st_t):
         alone = gpu_from_case(fc)
-        s1 = alone.relax_ex(criterion=CRIT_REL_RESIDUAL, tolerance=1e-9, omega=-1.0)
+        s1 = alone.relax_ex(criterion=CRIT_REL_RESIDUAL, tolerance=1e-9, omega=-1.0,
+                            tile_shape=s["tile_shape"])
         assert s["sweeps"] == s1["sweeps"]
         assert bits_equal(g.download(), alone.download())
