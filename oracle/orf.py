@@ -205,8 +205,10 @@ class OracleField:
             raise RelaxError(f"residual {rel_tol} not reached in {max_sweeps} sweeps")
         return int(done.value)
 
-    def sor_sweep(self, omega, threads=1):
-        span = self.free_span()
+    def sor_sweep(self, omega, threads=1, span=None):
+        """One red-black sweep over the free span (pass `span`, from
+        free_span(), to skip finding it again when the holds do not change)."""
+        span = self.free_span() if span is None else span
         s = (C.c_uint32 * 6)(*span)
         return lib().orf_sor_sweep(self._h, omega, s, threads)
 
