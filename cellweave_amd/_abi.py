@@ -92,6 +92,7 @@ def _declare():
         "cw_field_upload": (st, [P, dp, u8p, dp, dp]),
         "cw_field_download": (st, [P, dp]),
         "cw_field_relax": (st, [P, d, u64, u64p]),
+        "cw_field_relax_noalloc": (st, [P, d, u64, u64p]),
         "cw_field_relax_ex": (st, [P, opts, stats]),
         "cw_field_residual": (st, [P, dp]),
         "cw_batch_relax": (st, [C.POINTER(P), u32, opts, stats]),

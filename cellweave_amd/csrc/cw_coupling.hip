@@ -834,7 +834,7 @@ cw_status build_csr(cw_lattice* l, uint32_t ncells) {
 // ---- shared with the solve ------------------------------------------------------
 namespace cwi {
 
-cw_status build_coefficients(cw_field* f) {
+cw_status build_coefficients(cw_field* f, bool may_alloc) {
     const Geo g = geo_of(f);
     const unsigned gb = grid_of(f->npad);
     hipLaunchKernelGGL(k_scan_init, dim3(1), dim3(64), 0, f->stream, f->scan);
@@ -852,6 +852,10 @@ cw_status build_coefficients(cw_field* f) {
     for (int b = 0; b < 4; b++) sc[b] = (raw[10] >> b) & 1;
     const bool general = f->source != 0.0 || sc[0];
     if (general && !f->k2) {
+        if (!may_alloc)
+            return fail(CW_ERR_ARG,
+                        "the field keeps production and uptake apart and has no buffer for it "
+                        "yet; an allocating call (cw_field_relax) creates it");
         cw_status st = alloc_fill((void**)&f->k2, f->npad * sizeof(double), f->stream, 0);
         if (st) return st;
     }

@@ -43,6 +43,7 @@ constexpr int kRedSlot = 8;
 // field's state; the solve's coefficient words are derived from them.
 struct cw_field {
     uint32_t W = 0, H = 0, D = 0;
+    int device = 0;            // HIP device the field's memory lives on
     int P = 0;
     size_t n = 0, npad = 0;
     double diff = 0, decay = 0, source = 0;
@@ -75,7 +76,10 @@ namespace cwi {
 
 // Derive k (and k2), the sink flag and the free-span box from the device
 // state (cw_coupling.hip). Runs before a solve whenever the state changed.
-cw_status build_coefficients(cw_field* f);
+// Without may_alloc it fails (CW_ERR_ARG) instead of allocating the
+// general-form coefficient buffer a field needs the first time it keeps
+// production and uptake apart.
+cw_status build_coefficients(cw_field* f, bool may_alloc = true);
 // Flat (W*H*D) host copies of the device state.
 cw_status download_held(const cw_field* f, uint8_t* out);
 cw_status download_f64(const cw_field* f, const double* dev, double* out);

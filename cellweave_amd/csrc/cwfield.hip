@@ -358,6 +358,11 @@ struct Plan {
     bool compact = false;
     std::vector<char> act;
     int nact = 0;
+    // Arena mode: the device buffers are carved out of a region the plan does
+    // not own (the allocation-free entry point's scratch) instead of being
+    // allocated; plan_upload fails if the layout does not fit.
+    unsigned char* arena = nullptr;
+    size_t arena_bytes = 0;
 
     // Back to a fresh plan, keeping the device buffers and their capacities
     // (and dropping a captured graph, whose launch grids and arguments belong
@@ -380,6 +385,7 @@ struct Plan {
 
     ~Plan() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (arena) return;  // nothing here was allocated by the plan
         (void)hipFree(run);
         (void)hipFree(dp); (void)hipFree(dc); (void)hipFree(partials);
         for (int i = 0; i < 3; i++) (void)hipFree(dtp[i]);
@@ -458,18 +464,40 @@ cw_status plan_upload(Plan& pl, const std::vector<char>* active = nullptr) {
         }
     }
     cw_status st;
-    if ((st = grow(&pl.dp, &pl.cap_prob, (size_t)std::max(1, np)))) return st;
-    if ((st = grow(&pl.dc, &pl.cap_ctl, (size_t)std::max(1, np)))) return st;
     size_t maxt = 1;
+    for (int tl = 0; tl < 3; tl++) maxt = std::max(maxt, pl.tp[tl].size());
+    pl.part_stride = (int)(2 * maxt);
+    if (pl.arena) {
+        // problems, control blocks, the three tile tables and the partials,
+        // each on a 256-byte boundary
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            unsigned char* p = pl.arena + off;
+            off += (bytes + 255) / 256 * 256;
+            return p;
+        };
+        pl.dp = (Prob*)take(std::max(1, np) * sizeof(Prob));
+        pl.dc = (Ctl*)take(std::max(1, np) * sizeof(Ctl));
+        for (int tl = 0; tl < 3; tl++) pl.dtp[tl] = (int*)take(std::max<size_t>(1, pl.tp[tl].size()) * sizeof(int));
+        pl.partials = (double*)take(2 * (size_t)pl.part_stride * sizeof(double));
+        if (off > pl.arena_bytes)
+            return fail(CW_ERR_ARG,
+                        "the solve needs %zu bytes of launch-plan scratch; the allocation-free "
+                        "entry point holds %zu",
+                        off, pl.arena_bytes);
+    } else {
+        if ((st = grow(&pl.dp, &pl.cap_prob, (size_t)std::max(1, np)))) return st;
+        if ((st = grow(&pl.dc, &pl.cap_ctl, (size_t)std::max(1, np)))) return st;
+        for (int tl = 0; tl < 3; tl++)
+            if (!pl.tp[tl].empty() && (st = grow(&pl.dtp[tl], &pl.cap_tp[tl], pl.tp[tl].size())))
+                return st;
+        if ((st = grow(&pl.partials, &pl.cap_part, 2 * (size_t)pl.part_stride))) return st;
+    }
     for (int tl = 0; tl < 3; tl++) {
-        maxt = std::max(maxt, pl.tp[tl].size());
         if (pl.tp[tl].empty()) continue;
-        if ((st = grow(&pl.dtp[tl], &pl.cap_tp[tl], pl.tp[tl].size()))) return st;
         CW_HIP(hipMemcpyAsync(pl.dtp[tl], pl.tp[tl].data(), pl.tp[tl].size() * sizeof(int),
                               hipMemcpyHostToDevice, pl.stream));
     }
-    pl.part_stride = (int)(2 * maxt);
-    if ((st = grow(&pl.partials, &pl.cap_part, 2 * (size_t)pl.part_stride))) return st;
     CW_HIP(hipMemcpyAsync(pl.dp, pl.hp.data(), np * sizeof(Prob), hipMemcpyHostToDevice,
                           pl.stream));
     CW_HIP(hipMemcpyAsync(pl.dc, pl.hc.data(), np * sizeof(Ctl), hipMemcpyHostToDevice, pl.stream));
@@ -921,6 +949,9 @@ struct Opts {
     // GPU, so the plane tile shape is chosen for their tiles together)
     cw_field* const* peers = nullptr;
     uint32_t npeers = 0;
+    // allocation-free solve: the launch plan lives in this region (no graphs)
+    unsigned char* arena = nullptr;
+    size_t arena_bytes = 0;
 };
 
 // Times n sweep launches of the plan on its stream with HIP events (the
@@ -1010,7 +1041,9 @@ cw_status solve_fp64(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_st
     int s = o.s ? std::min(o.s, kMaxS) : 0;  // 0: the library's choice
     if (three_d) s = 1;
     Plan local;
-    Plan& pl = nf == 1 ? field_plan(F[0], 0) : local;
+    Plan& pl = (nf == 1 && !o.arena) ? field_plan(F[0], 0) : local;
+    pl.arena = o.arena;
+    pl.arena_bytes = o.arena_bytes;
     pl.stream = stream;
     pl.fp64 = true;
     pl.three_d = three_d;
@@ -1428,14 +1461,16 @@ cw_status solve_concurrently(cw_field* const* F, uint32_t nf, const Opts& o,
 }
 
 cw_status solve_impl(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts,
-                     cw_relax_stats* stats, long time_passes, double* time_ms);
+                     cw_relax_stats* stats, long time_passes, double* time_ms,
+                     unsigned char* arena, size_t arena_bytes);
 
 // Every solve entry point comes through here: no C++ exception crosses the
 // C ABI (host allocations for plans and threads can throw).
 cw_status solve(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts, cw_relax_stats* stats,
-                long time_passes = 0, double* time_ms = nullptr) {
+                long time_passes = 0, double* time_ms = nullptr, unsigned char* arena = nullptr,
+                size_t arena_bytes = 0) {
     try {
-        return solve_impl(F, nf, opts, stats, time_passes, time_ms);
+        return solve_impl(F, nf, opts, stats, time_passes, time_ms, arena, arena_bytes);
     } catch (const std::bad_alloc&) {
         return fail(CW_ERR_DEVICE, "out of host memory");
     } catch (const std::exception& e) {
@@ -1444,7 +1479,8 @@ cw_status solve(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts, cw_r
 }
 
 cw_status solve_impl(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts,
-                     cw_relax_stats* stats, long time_passes, double* time_ms) {
+                     cw_relax_stats* stats, long time_passes, double* time_ms,
+                     unsigned char* arena, size_t arena_bytes) {
     auto t0 = std::chrono::steady_clock::now();
     if (!F || nf == 0) return fail(CW_ERR_ARG, "no fields");
     Opts o;
@@ -1452,9 +1488,13 @@ cw_status solve_impl(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts,
     if (st) return st;
     o.time_passes = time_passes;
     o.time_ms = time_ms;
+    o.arena = arena;
+    o.arena_bytes = arena_bytes;
+    if (arena && (nf != 1 || o.precision != CW_PREC_FP64 || o.use_graph || time_passes))
+        return fail(CW_ERR_ARG, "the allocation-free solve is one FP64 field without graphs");
     for (uint32_t i = 0; i < nf; i++) {
         if (!F[i]) return fail(CW_ERR_ARG, "null field %u", i);
-        if (F[i]->coef_dirty && (st = build_coefficients(F[i]))) return st;
+        if (F[i]->coef_dirty && (st = build_coefficients(F[i], arena == nullptr))) return st;
     }
     std::vector<cw_relax_stats> local;
     if (!stats) {
@@ -1699,6 +1739,46 @@ cw_status field_adjoint(cw_field* f, const double* target, const uint32_t* grid,
     return CW_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free relax (cw_field_relax_noalloc): the launch
+// plan of one field (problems, control blocks, tile tables, residual
+// partials) lives here instead of in allocated device memory. Every GPU the
+// library runs on holds one copy until the process ends, and every call on
+// that GPU shares it: two such calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = (size_t)4 << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_addr[kMaxDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+// The device address of `scratch` on device dev, looked up once per device
+// with that device current. Host code uses only this address: the array's
+// own name is a host-side stand-in, not device memory.
+cw_status scratch_on(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(CW_ERR_DEVICE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        int cur = 0;
+        CW_HIP(hipGetDevice(&cur));
+        if (cur != dev) CW_HIP(hipSetDevice(dev));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(CW_ERR_DEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                        hipGetErrorString(e));
+        g_scratch_addr[dev] = (unsigned char*)p;
+    }
+    *out = g_scratch_addr[dev];
+    return CW_OK;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1747,6 +1827,10 @@ cw_status cw_field_create(uint32_t w, uint32_t h, uint32_t d, double diff, doubl
     cw_status st = device_init();
     if (st) return st;
     cw_field* f = new cw_field();
+    if (hipGetDevice(&f->device) != hipSuccess) {
+        delete f;
+        return fail(CW_ERR_DEVICE, "hipGetDevice failed");
+    }
     f->W = w; f->H = h; f->D = d;
     f->P = (int)((w + 63) / 64 * 64);
     f->n = (size_t)w * h * d;
@@ -1837,8 +1921,27 @@ cw_status cw_field_relax(cw_field* f, double tolerance, uint64_t max_sweeps, uin
     // warm-started relax of every lattice step launches once per 16 sweeps
     // instead of once per sweep. Same sweeps and bits as eager launches.
     o.use_graph = 1;
-    cw_relax_stats s;
+    cw_relax_stats s{};
     cw_status st = solve(&f, 1, &o, &s);
+    if (sweeps) *sweeps = s.sweeps;
+    return st;
+}
+
+cw_status cw_field_relax_noalloc(cw_field* f, double tolerance, uint64_t max_sweeps,
+                                 uint64_t* sweeps) {
+    if (!f) return fail(CW_ERR_ARG, "null field");
+    unsigned char* sc = nullptr;
+    cw_status st = scratch_on(f->device, &sc);
+    if (st) return st;
+    // cw_field_relax's options, with eager launches: instantiating a graph
+    // allocates
+    cw_relax_opts o;
+    cw_relax_opts_default(&o);
+    o.tolerance = tolerance;
+    o.max_sweeps = max_sweeps;
+    o.use_graph = 0;
+    cw_relax_stats s{};
+    st = solve(&f, 1, &o, &s, 0, nullptr, sc, kNoAllocScratchBytes);
     if (sweeps) *sweeps = s.sweeps;
     return st;
 }

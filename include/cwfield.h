@@ -133,6 +133,21 @@ cw_status cw_field_download(const cw_field *field, double *values);
 cw_status cw_field_relax(cw_field *field, double tolerance, uint64_t max_sweeps,
                          uint64_t *sweeps);
 
+/* cw_field_relax that allocates nothing (no counterpart in the reference; it
+ * is for callers that cannot allocate device memory: tight serving loops and
+ * harnesses that forbid hipMalloc). Same semantics, kernels and bits as
+ * cw_field_relax, with eager launches instead of graph replays. The launch
+ * plan (problems, control blocks, tile tables, residual partials) lives in a
+ * static 4 MiB device array, one per GPU, kept until the process ends; a
+ * field whose plan needs more (about 9e4 elementwise tiles of 128 x 4 voxels,
+ * e.g. beyond 400^3) gets CW_ERR_ARG before any sweep. So does a field that
+ * keeps production and uptake apart (a source, or voxels with both) before an
+ * allocating call has created its buffer for that. Calls on one GPU share the
+ * array: two calls that overlap on one GPU race. The verdicts are read on the
+ * host, so it cannot be captured into a graph. */
+cw_status cw_field_relax_noalloc(cw_field *field, double tolerance, uint64_t max_sweeps,
+                                 uint64_t *sweeps);
+
 /* relax with every knob: omega, criterion, precision, blocking depth. */
 cw_status cw_field_relax_ex(cw_field *field, const cw_relax_opts *opts,
                             cw_relax_stats *stats);
