@@ -96,6 +96,8 @@ def _declare():
         "cw_field_relax_ex": (st, [P, opts, stats]),
         "cw_field_residual": (st, [P, dp]),
         "cw_batch_relax": (st, [C.POINTER(P), u32, opts, stats]),
+        "cw_campaign_relax": (st, [C.POINTER(P), u32, u32p, C.POINTER(C.c_int32), u32, opts,
+                                   stats]),
         "cw_field_fill": (st, [P, d]),
         "cw_batch_time_passes": (st, [C.POINTER(P), u32, opts, u32, dp]),
         "cw_field_adjoint": (st, [P, dp, u32p, u32, opts, dp, C.POINTER(AdjointResult)]),

@@ -2,11 +2,13 @@
 
 The reference runs a campaign's histories in parallel on CPU threads, each with
 its own fields (src/campaign.rs:196-204). Here every field of every history is
-a problem in one batched launch sequence (cw_batch_relax): each launch sweeps
-the tiles of all unconverged fields, and each field stops on its own criterion.
-Fields are grouped by size class, each class with its own tile shape from the
-autotuner table, and steady chunks of passes are replayed from a captured
-hipGraph.
+solved by one device-driven launch sequence (cw_campaign_relax): fields are
+grouped by size class, each class swept with its own tile shape from the
+autotuner table; every field keeps its own state on the device, a small kernel
+between steps moves it to its next phase (FP64 sweeps and residual tests, or
+the FP32 refinement's residual, smoothing segment and correction), and one
+captured hipGraph holds every class's chain of steps. The host launches the
+graph and waits once.
 
     python -m cellweave_amd.campaign --tune     # on a GPU: writes the shape table
 """
@@ -19,12 +21,15 @@ import os
 import time
 
 from .field import (CRIT_REL_RESIDUAL, PREC_FP32_REFINE, PREC_FP64, Field, autotune,
-                    batch_relax)
+                    batch_relax, campaign_relax)
 
 TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_gfx950.json")
 
 # Size classes by the longest grid edge.
 CLASSES = ((0, 256, "<256"), (256, 512, "256-511"), (512, 1 << 30, ">=512"))
+
+# Plane tile shapes the device-driven batch is compiled for (cw_campaign_relax).
+DEVICE_SHAPES = {"fp64": (8, 9), "fp32": (15, 18, 19, 20, 21, 22, 23, 24)}
 
 
 def size_class(n: int) -> str:
@@ -76,14 +81,27 @@ def plan(cases_, precision=PREC_FP32_REFINE, table=None):
 
 
 def solve(fields, plan_, precision=PREC_FP32_REFINE, tol=1e-8, use_graph=1,
-          max_sweeps=2_000_000, omega=-1.0, concurrent=True):
-    """Solve every field to `tol`, one batched launch sequence per size class.
-    omega < 0 gives each field its own 2/(1+sin(pi/L)), L its longest free-span
-    edge. With `concurrent` the classes run at once, one thread each: a batch
-    runs on its first field's stream, so the classes' launches overlap on the
-    GPU (the library releases the interpreter lock while it works). Returns
-    (stats per field, seconds)."""
+          max_sweeps=2_000_000, omega=-1.0, concurrent=True, device_driven=True):
+    """Solve every field to `tol`. omega < 0 gives each field its own
+    2/(1+sin(pi/L)), L its longest free-span edge. Returns (stats per field,
+    seconds).
+
+    device_driven (the default): one cw_campaign_relax call for every class,
+    each with its plan's tile shape. Otherwise one cw_batch_relax launch
+    sequence per size class, verdicts read by the host; with `concurrent` the
+    classes run at once, one host thread each."""
     stats = [None] * len(fields)
+    if device_driven:
+        groups = [0] * len(fields)
+        shapes = []
+        for g, (_cls, idx, shape) in enumerate(plan_):
+            for k in idx:
+                groups[k] = g
+            shapes.append(shape)
+        t0 = time.perf_counter()
+        st, _ = campaign_relax(fields, groups, shapes, criterion=CRIT_REL_RESIDUAL, tolerance=tol,
+                               max_sweeps=max_sweeps, omega=omega, precision=precision)
+        return st, time.perf_counter() - t0
 
     def one(entry):
         cls, idx, shape = entry
@@ -127,8 +145,11 @@ def tune(cases_, per_class=0, launches=10, confirm=4, tol=1e-8):
             ranked = autotune(fields, launches=launches, criterion=CRIT_REL_RESIDUAL,
                               tolerance=tol, omega=-1.0, precision=prec)
             reset()
+            reset()
+            # the campaign runs device-driven: only the shapes compiled for it
+            runnable = [r for r in ranked if r[0] in DEVICE_SHAPES[key]]
             timed = []
-            for shape, _ in ranked[:max(1, confirm)]:
+            for shape, _ in runnable[:max(1, confirm)]:
                 best = None
                 for _rep in range(2):  # the first also builds the plan and graph
                     reset()
@@ -138,7 +159,7 @@ def tune(cases_, per_class=0, launches=10, confirm=4, tol=1e-8):
                 timed.append([shape, 1e3 * best])
             timed.sort(key=lambda e: e[1])
             table[key][cls] = timed[0][0]
-            table["ranking"][f"{key} {cls}"] = ranked[:6]
+            table["ranking"][f"{key} {cls}"] = runnable[:6]
             table["confirmed_ms"][f"{key} {cls}"] = timed
             reset()
     return table

@@ -113,7 +113,7 @@ __device__ __forceinline__ PassArgs resolve(const PassArgs& a0) {
     return a;
 }
 
-__global__ void advance_run(int* run, int passes, int sweeps) {
+static __global__ void advance_run(int* run, int passes, int sweeps) {
     if (threadIdx.x == 0) {
         run[0] += passes;
         run[1] += sweeps;
@@ -875,259 +875,31 @@ __device__ __forceinline__ void reg_half(RegCells<T, NR>& R, T above, T below, i
     }
 }
 
+// The register-blocked sweep's workgroup body lives in cw_relax2d_tile.inc:
+// the kernel below and the device-driven batch's tile function (the tile and
+// its input buffer given, no launch prologue) each expand it in place.
 template <typename T, int S, int NW, int NR, bool PLAIN, bool GEN>
 __global__ __launch_bounds__(NW * 64) void relax2d_reg(PassArgs a_in) {
     const PassArgs a = resolve(a_in);
-    constexpr int H = 2 * S;
-    constexpr int RX = 128;
-    constexpr int RY = NW * NR;
-    constexpr int TX = RX - 2 * H;
-    constexpr int TY = RY - 2 * H;
-    static_assert(TY > 0 && NR >= 2 && S <= 16, "region shape");
-    static_assert(!PLAIN || S <= kMaxS, "change slots");
-    using T2 = typename V2<T>::type;
-    __shared__ T edge[2][NW][2][2][64];  // [slot][wave][first/last row][colour][lane]
+#define CW_TILE_INDEX xcd_remap(blockIdx.x, gridDim.x)
+#define CW_INPUT_PARITY (a.pass & 1)
+#define CW_TILE_PROLOGUE 1
+#include "cw_relax2d_tile.inc"
+#undef CW_TILE_INDEX
+#undef CW_INPUT_PARITY
+#undef CW_TILE_PROLOGUE
+}
 
-    const int lane = threadIdx.x & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int t = xcd_remap(blockIdx.x, gridDim.x);
-    const int p = a.tile_prob[t];
-    const Prob& pr = a.probs[p];
-    const int tl = a.tl;
-    const int lt = t - pr.sw_base[tl];
-    const int ti = lt % pr.ntx[tl], tj = lt / pr.ntx[tl];
-    const int gxo = pr.xs + ti * TX - H;
-    const int gyo = pr.ys + tj * TY - H;
-    const int W = pr.W, Hh = pr.H, P = pr.P;
-    const int gx = gxo + 2 * lane;
-    const bool colin = gx >= 0 && gx < W;
-    const T* kk = (const T*)pr.k;
-    const T* k2p = (const T*)(PLAIN ? pr.k2 : pr.lb);
-    // PLAIN: GEN keeps production and uptake apart. Smoother: GEN means the
-    // launch carries lower bounds (a field without one has none to load).
-    const bool has_k2 = PLAIN ? GEN : (GEN && k2p != nullptr);
-    const bool replay = a.mode == MODE_REPLAY;
-    const T d = (T)pr.d, decay = (T)pr.decay, omega = (T)pr.omega, omo = (T)pr.omo;
-    const T source = (T)pr.source;
-    const int r0 = wv * NR;
-    const int base = (gxo + gyo + r0) & 1;  // colour of column 2l on the wave's first row
-    const int rx[2] = {(gx > 0) + (gx < W - 1), (gx + 1 > 0) + (gx + 1 < W - 1)};
-    T al[2][3];
-    if (!PLAIN) {
-        const T od = omega * d;
-#pragma unroll
-        for (int q = 0; q < 2; q++)
-#pragma unroll
-            for (int ry = 0; ry < 3; ry++) al[q][ry] = od * rcp_t((T)(rx[q] + ry) * d + decay);
-    }
-
-    RegCells<T, NR> R;
-    // Issue every row's loads before using any: a row outside the field
-    // reads element 0 (always in bounds) and is masked below, so no load
-    // waits behind another row's branch.
-    T2 lc[NR], lu[NR], lv[NR];
-    const T* xin = (const T*)pr.x[a.pass & 1];
-#pragma unroll
-    for (int i = 0; i < NR; i++) {
-        const int gy = gyo + r0 + i;
-        const bool in = colin && gy >= 0 && gy < Hh;
-        const size_t off = in ? (size_t)gy * P + gx : 0;
-        lc[i] = *(const T2*)(kk + off);
-        if (has_k2) lu[i] = *(const T2*)(k2p + off);
-        if (!replay) lv[i] = *(const T2*)(xin + off);
-    }
-    // Decide whether this launch runs while the tile loads are in flight: the
-    // verdict's own reads then overlap them instead of following them.
-    int par;
-    if (!sweep_prologue(a, pr, p, lt, &par)) return;
-#pragma unroll
-    for (int i = 0; i < NR; i++) {
-        const int gy = gyo + r0 + i;
-        T ve = T(0), vo = T(0), ce = nan_of<T>(), co = nan_of<T>(), ue = T(0), uo = T(0);
-        if (colin && gy >= 0 && gy < Hh) {
-            ce = lc[i].x; co = lc[i].y;
-            if (has_k2) { ue = lu[i].x; uo = lu[i].y; }
-            if (!replay) { ve = lv[i].x; vo = lv[i].y; }
-        }
-        // colour of the even column on row i is base ^ (i & 1)
-        const bool sw = (base ^ (i & 1)) != 0;
-        R.st[0][i] = sw ? vo : ve; R.st[1][i] = sw ? ve : vo;
-        R.kc[0][i] = sw ? co : ce; R.kc[1][i] = sw ? ce : co;
-        R.k2[0][i] = sw ? uo : ue; R.k2[1][i] = sw ? ue : uo;
-        if (!PLAIN) {
-            const int ry = (gy > 0) + (gy < Hh - 1);
-#pragma unroll
-            for (int c = 0; c < 2; c++) {
-                const int q = (c ^ base ^ (i & 1)) & 1;  // x parity of colour c on row i
-                const T b = R.kc[c][i];
-                const bool fr = b == b;
-                R.al[c][i] = fr ? al[q][ry] : T(0);
-                R.ga[c][i] = fr ? -omega : T(0);
-                R.kc[c][i] = fr ? b : T(0);
-                if (!has_k2) R.k2[c][i] = neg_inf<T>();
-            }
-        }
-    }
-
-    const T* src = (const T*)pr.x[par];
-    T* dst = (T*)pr.x[par ^ 1];
-    if (replay) {
-#pragma unroll
-        for (int i = 0; i < NR; i++) {
-            const int gy = gyo + r0 + i;
-            if (colin && gy >= 0 && gy < Hh) {
-                const T2 v = *(const T2*)(src + (size_t)gy * P + gx);
-                const bool sw = (base ^ (i & 1)) != 0;
-                R.st[0][i] = sw ? v.y : v.x; R.st[1][i] = sw ? v.x : v.y;
-            }
-        }
-    }
-
-    // Residual of the input on the interior (PLAIN, relative-residual rule).
-    double r2 = 0.0;
-    if (PLAIN && a.check) {
-        edge[0][wv][0][0][lane] = R.st[0][0];
-        edge[0][wv][0][1][lane] = R.st[1][0];
-        edge[0][wv][1][0][lane] = R.st[0][NR - 1];
-        edge[0][wv][1][1][lane] = R.st[1][NR - 1];
-        __syncthreads();
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            const T above = wv > 0 ? edge[0][wv - 1][1][c ^ 1][lane] : T(0);
-            const T below = wv < NW - 1 ? edge[0][wv + 1][0][c ^ 1][lane] : T(0);
-#pragma unroll
-            for (int i = 0; i < NR; i++) {
-                const int r = r0 + i;
-                if (r < H || r >= H + TY) continue;
-                const int q = (c ^ base ^ (i & 1)) & 1;
-                const int x = 2 * lane + q;
-                const T kq = R.kc[c][i];
-                const T o = R.st[c ^ 1][i];
-                const T fl = from_left(o), fr = from_right(o);
-                if (x < H || x >= H + TX || kq != kq) continue;
-                const T across = q ? fr : fl;
-                const T up = i > 0 ? R.st[c ^ 1][i - 1] : above;
-                const T dn = i < NR - 1 ? R.st[c ^ 1][i + 1] : below;
-                T sum = T(0);
-                if (q == 0) { sum += across; sum += o; } else { sum += o; sum += across; }
-                sum += up;
-                sum += dn;
-                const int gy = gyo + r;
-                const int ry = (gy > 0) + (gy < Hh - 1);
-                const double A = (double)(rx[q] + ry) * (double)d + (double)decay;
-                const double b = GEN ? (((double)source + (double)kq) - (double)R.k2[c][i])
-                                     : (double)kq;
-                const double cc = (double)R.st[c][i];
-                double rr = (b + (double)d * (double)sum) - A * cc;
-                if (A > 0.0 && !(cc + rr / A > 0.0)) rr = -A * cc;
-                r2 += rr * rr;
-            }
-        }
-        __syncthreads();
-    }
-
-    edge[0][wv][0][0][lane] = R.st[0][0];
-    edge[0][wv][0][1][lane] = R.st[1][0];
-    edge[0][wv][1][0][lane] = R.st[0][NR - 1];
-    edge[0][wv][1][1][lane] = R.st[1][NR - 1];
-    __syncthreads();
-
-    unsigned long long mv[S > kMaxS ? 1 : S];
-#pragma unroll
-    for (int j = 0; j < (S > kMaxS ? 1 : S); j++) mv[j] = 0ull;
-    const bool has_lb = !PLAIN && has_k2;
-    // The half-sweeps, compiled once per colour parity of row 0 (B), so each
-    // is straight-line code with no branch on the region's position and no
-    // register copies where two such branches would join.
-    if constexpr (PLAIN) {
-        // FP64: unrolled, each sweep's change to its own slot
-        auto sweeps = [&](auto base_c) {
-            constexpr int B = decltype(base_c)::value;
-#pragma unroll
-            for (int h = 0; h < 2 * S; h++) {
-                const int slot = h & 1;
-                const int lo = h + 1, hiY = RY - 2 - h, hiX = RX - 2 - h;
-                // rows above and below hold the other colour at the same column
-                const int oc = (h & 1) ^ 1;
-                const T above = wv > 0 ? edge[slot][wv - 1][1][oc][lane] : T(0);
-                const T below = wv < NW - 1 ? edge[slot][wv + 1][0][oc][lane] : T(0);
-                unsigned long long* m = &mv[(h >> 1) % (S > kMaxS ? 1 : S)];
-                const int p0 = ((h & 1) ^ B) & 1;  // x parity of this colour on row 0
-                if (h & 1) {
-                    if (p0) reg_half<T, PLAIN, GEN, NR, 1, 1, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
-                    else reg_half<T, PLAIN, GEN, NR, 1, 0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
-                } else {
-                    if (p0) reg_half<T, PLAIN, GEN, NR, 0, 1, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
-                    else reg_half<T, PLAIN, GEN, NR, 0, 0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, m);
-                }
-                if (h + 1 < 2 * S) {
-                    const int ns = slot ^ 1;
-                    edge[ns][wv][0][0][lane] = R.st[0][0];
-                    edge[ns][wv][0][1][lane] = R.st[1][0];
-                    edge[ns][wv][1][0][lane] = R.st[0][NR - 1];
-                    edge[ns][wv][1][1][lane] = R.st[1][NR - 1];
-                    __syncthreads();
-                }
-            }
-        };
-        if (base) sweeps(std::integral_constant<int, 1>{});
-        else sweeps(std::integral_constant<int, 0>{});
-    } else {
-        // FP32 smoother: one sweep (two half-sweeps) per loop iteration. Its
-        // code stays small (two fully unrolled copies of 24 half-sweeps ran
-        // slower), and nothing is scheduled across a barrier anyway.
-        auto sweeps = [&](auto base_c) {
-            constexpr int B = decltype(base_c)::value;
-            auto half = [&](auto colour_c, int h) {
-                constexpr int HC = decltype(colour_c)::value;  // colour updated, h & 1
-                constexpr int P0 = HC ^ B;  // x parity of this colour on row 0
-                const int lo = h + 1, hiY = RY - 2 - h, hiX = RX - 2 - h;
-                // Rows above and below hold the other colour at the same
-                // column. The first and last waves read their own edge
-                // instead: it feeds only rows 0 and RY-1, which are outside
-                // [lo, hiY] at every depth. The unconditional read keeps the
-                // half-sweep one basic block, so the DPP shifts fuse into adds.
-                const T above = edge[HC][wv > 0 ? wv - 1 : wv][1][HC ^ 1][lane];
-                const T below = edge[HC][wv < NW - 1 ? wv + 1 : wv][0][HC ^ 1][lane];
-                reg_half<T, PLAIN, GEN, NR, HC, P0, S>(R, above, below, r0, lo, hiY, hiX, lane, gyo, Hh, rx, d, decay, omega, omo, source, al, has_lb, H, TX, TY, &mv[0]);
-                if (h + 1 < 2 * S) {
-                    edge[HC ^ 1][wv][0][0][lane] = R.st[0][0];
-                    edge[HC ^ 1][wv][0][1][lane] = R.st[1][0];
-                    edge[HC ^ 1][wv][1][0][lane] = R.st[0][NR - 1];
-                    edge[HC ^ 1][wv][1][1][lane] = R.st[1][NR - 1];
-                    __syncthreads();
-                }
-            };
-#pragma unroll 1
-            for (int h = 0; h < 2 * S; h += 2) {
-                half(std::integral_constant<int, 0>{}, h);
-                half(std::integral_constant<int, 1>{}, h + 1);
-            }
-        };
-        if (base) sweeps(std::integral_constant<int, 1>{});
-        else sweeps(std::integral_constant<int, 0>{});
-    }
-
-    if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= pr.x1) {
-#pragma unroll
-        for (int i = 0; i < NR; i++) {
-            const int r = r0 + i;
-            if (r < H || r >= H + TY) continue;
-            const int gy = gyo + r;
-            if (gy > pr.y1) continue;
-            const bool sw = (base ^ (i & 1)) != 0;
-            T2 v;
-            v.x = sw ? R.st[1][i] : R.st[0][i];
-            v.y = sw ? R.st[0][i] : R.st[1][i];
-            *(T2*)(dst + (size_t)gy * P + gx) = v;
-        }
-    }
-
-    if (PLAIN) {
-        if (a.check) block_partial(a.partials + (a.pass & 1) * a.part_stride, t, r2, 0.0);
-        if (a.crit == CRIT_MAXCHANGE && a.mode == MODE_SWEEP)
-            block_moved<NW, (S > kMaxS ? 1 : S)>(a.ctl[p], a.pass, mv);
-    }
+// One tile of the device-driven batch: tile t, input in buffer in_par.
+template <typename T, int S, int NW, int NR, bool PLAIN, bool GEN>
+__device__ __forceinline__ void relax2d_reg_tile(const PassArgs& a, const int t_in, const int in_par) {
+#define CW_TILE_INDEX t_in
+#define CW_INPUT_PARITY in_par
+#define CW_TILE_PROLOGUE 0
+#include "cw_relax2d_tile.inc"
+#undef CW_TILE_INDEX
+#undef CW_INPUT_PARITY
+#undef CW_TILE_PROLOGUE
 }
 
 // ---------------------------------------------------------------------------
@@ -1618,7 +1390,7 @@ __global__ __launch_bounds__(256) void half3d(PassArgs a_in, int colour) {
 // residual; b_eff adds the held neighbours. With write32 it also writes the
 // FP32 smoother's right-hand side omega*r/A, the correction's lower bound -c
 // (rounded toward zero) and zeroes e[0] for the FP32 inner solve.
-__global__ __launch_bounds__(256) void resid64(PassArgs a, int write32) {
+static __global__ __launch_bounds__(256) void resid64(PassArgs a, int write32) {
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int p = a.tile_prob[t];
     const Prob& pr = a.probs[p];
@@ -1679,7 +1451,7 @@ __global__ __launch_bounds__(256) void resid64(PassArgs a, int write32) {
 }
 
 // c <- max(c + e, 0) on the free cells, e from the buffer the inner plan ended in.
-__global__ __launch_bounds__(256) void correct64(PassArgs a) {
+static __global__ __launch_bounds__(256) void correct64(PassArgs a) {
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int p = a.tile_prob[t];
     const Prob& pr = a.probs[p];
@@ -1759,7 +1531,7 @@ constexpr int kDecideShared = 4;  // up to this many problems: the whole block s
 // problems are summed by the whole block in turn (a large field has tens of
 // thousands of partials); many are summed one per wave. Either way each sum
 // has a fixed order.
-__global__ __launch_bounds__(kDecideThreads) void decide(PassArgs a_in) {
+static __global__ __launch_bounds__(kDecideThreads) void decide(PassArgs a_in) {
     const PassArgs a = resolve(a_in);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     if (a.mode != DECIDE_REPLAY_END && a.nprob <= kDecideShared) {
@@ -1823,7 +1595,7 @@ __global__ __launch_bounds__(kDecideThreads) void decide(PassArgs a_in) {
 // Adjoint set-up over the swept box: rhs = c - target and the smoother's
 // pre-scaled beta = omega * rhs / A on free cells (NaN on held ones), and
 // lambda = 0 in both buffers.
-__global__ __launch_bounds__(256) void adjoint_setup(PassArgs a, const double* c,
+static __global__ __launch_bounds__(256) void adjoint_setup(PassArgs a, const double* c,
                                                      const double* target, double* rhs,
                                                      double* beta, double* lam0, double* lam1) {
     const int t = xcd_remap(blockIdx.x, gridDim.x);
@@ -1858,7 +1630,7 @@ __global__ __launch_bounds__(256) void adjoint_setup(PassArgs a, const double* c
 // dL/dg from the free cells), for the closed forms
 //   dL/dD = sum_i lambda_i sum_{real nbrs j} (c_j - c_i),
 //   dL/dg = D sum_i lambda_i #held_nbrs(i)   (+ the held cells' own term).
-__global__ __launch_bounds__(256) void adjoint_grads(PassArgs a, const double* c,
+static __global__ __launch_bounds__(256) void adjoint_grads(PassArgs a, const double* c,
                                                      const double* lam, const double* kk) {
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int p = a.tile_prob[t];
@@ -1897,7 +1669,7 @@ __global__ __launch_bounds__(256) void adjoint_grads(PassArgs a, const double* c
 
 // dL/du_k = -sum of lambda over the voxels whose uptake is cell k's rate,
 // one thread per cell over a CSR list, in a fixed order.
-__global__ __launch_bounds__(256) void cell_sums(const double* lam, const long long* off,
+static __global__ __launch_bounds__(256) void cell_sums(const double* lam, const long long* off,
                                                  const long long* idx, int ncells,
                                                  double* out) {
     const int k = blockIdx.x * 256 + threadIdx.x;
@@ -1909,7 +1681,7 @@ __global__ __launch_bounds__(256) void cell_sums(const double* lam, const long l
 
 // Every cell of the box (not the row padding) set to v: ScalarField::fill
 // (pde.rs:212-215) on the device.
-__global__ __launch_bounds__(256) void fill_cells(double* c, int W, long rows, int P, double v) {
+static __global__ __launch_bounds__(256) void fill_cells(double* c, int W, long rows, int P, double v) {
     const long n = rows * (long)W;
     for (long i = blockIdx.x * 256l + threadIdx.x; i < n; i += (long)gridDim.x * 256l) {
         const long r = i / W, x = i - r * W;

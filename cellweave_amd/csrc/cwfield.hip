@@ -6,8 +6,11 @@
 // correction, and the batched solve. Kernels live in cw_kernels.hpp.
 #include "cw_kernels.hpp"
 #include "cw_internal.hpp"
+#include "cw_shapes.hpp"
+#include "cw_campaign.hpp"
 
 #include <algorithm>
+#include <memory>
 #include <chrono>
 #include <condition_variable>
 #include <cmath>
@@ -84,24 +87,7 @@ cw_status alloc_zero(void** p, size_t bytes, hipStream_t s, int fill) {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// Sweep-tile shapes: s sweeps per launch over regions 128 x RY cells with NW
-// waves; the interior is (128 - 4s) x (RY - 4s). kind 0: LDS tile kernel;
-// 1: register-blocked by column parity; 2: register-blocked by colour
-// (relax2d_reg, NR = RY / NW rows per wave). Index 0 of each table is the
-// one-sweep shape used for remainders and exact-stop replays.
-struct Shape { int s, ry, nw, kind; };
-constexpr Shape kPlainShapes[] = {{1, 32, 4, 0}, {2, 32, 4, 0},  {1, 32, 4, 2}, {2, 32, 4, 2},
-                                  {2, 64, 8, 2}, {2, 64, 16, 2}, {1, 32, 8, 2}, {4, 32, 4, 2},
-                                  {4, 64, 8, 2}, {8, 64, 8, 2}};
-constexpr Shape kSmoothShapes[] = {{1, 32, 4, 0},   {2, 32, 4, 0},   {4, 64, 8, 0},   {4, 64, 4, 0},
-                                   {8, 128, 8, 0},  {8, 128, 16, 0}, {2, 32, 4, 1},   {4, 64, 16, 1},
-                                   {4, 64, 8, 1},   {8, 128, 16, 1}, {2, 64, 8, 1},   {4, 128, 16, 1},
-                                   {2, 32, 4, 2},   {4, 64, 8, 2},   {4, 64, 16, 2},  {8, 128, 16, 2},
-                                   {2, 64, 8, 2},   {4, 128, 16, 2}, {6, 128, 16, 2},
-                                   {10, 128, 16, 2}, {12, 128, 16, 2}, {6, 64, 8, 2},
-                                   {8, 64, 8, 2},   {12, 64, 8, 2},  {16, 128, 16, 2}};
-constexpr int kNumPlainShapes = (int)(sizeof(kPlainShapes) / sizeof(kPlainShapes[0]));
-constexpr int kNumSmoothShapes = (int)(sizeof(kSmoothShapes) / sizeof(kSmoothShapes[0]));
+// The plane tile shapes (kPlainShapes, kSmoothShapes) are in cw_shapes.hpp.
 // 3D shapes: kind 0 = in-place red and black half sweeps (half3d, two
 // launches per sweep); kind 1 = z-streaming relax3d_stream with NW waves of
 // NR rows and zc planes per chunk (one launch per sweep, ping-pong); kind 2 =
@@ -1527,6 +1513,430 @@ cw_status solve_impl(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts,
     return CW_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Device-driven batch of planes (cw_campaign_relax; cw_campaign.hpp): each
+// group of fields (a size class) is one launch plan swept with its own tile
+// shape; every field's state lives on the device, and one captured graph
+// holds every group's chain of steps as parallel branches. The host launches
+// the graph as often as the model problem predicts, waits once and reads the
+// fields' states; only a field still running then costs another launch.
+// ---------------------------------------------------------------------------
+constexpr int kCampSteps = 64;  // rounds (steps of every group) in the captured graph
+
+struct CampGroup {
+    Plan pl;                      // sweep view of every problem, tiles, partials, control
+    std::vector<Prob> hp64;       // FP32 refinement: the FP64 view
+    std::vector<CampField> hst;   // per problem, host copy
+    std::vector<int> idx;         // plan problem -> field
+    unsigned char* dev = nullptr; // CampField[np] | Prob[np] | count | list[ntiles]
+    CampField* dst = nullptr;
+    Prob* dp64 = nullptr;
+    int* run = nullptr;           // the step base every group shares
+    int* count = nullptr;
+    int* list = nullptr;
+    int grid = 0, grid_tail = 0, shape = 0, s = 1;
+    std::vector<int> pred;        // per problem: the last step the model problem predicts
+    bool gen = false;
+    hipStream_t stream = nullptr;
+    ~CampGroup() { (void)hipFree(dev); }
+};
+
+CampArgs camp_args(CampGroup& g, int step, const Opts& o, int check_every, bool fp64) {
+    CampArgs ca{};
+    ca.pa = make_args(g.pl, TL_MAIN, MODE_SWEEP, CRIT_RESID, o.tol, 0);
+    ca.pa.pass = step;
+    ca.pa.s = g.s;
+    ca.pa.run = g.run;
+    ca.probs64 = g.dp64;
+    ca.st = g.dst;
+    ca.list = g.list;
+    ca.count = g.count;
+    ca.fp64 = fp64 ? 1 : 0;
+    ca.check_every = check_every;
+    ca.lo = 16;
+    ca.max_sweeps = (int)std::min<uint64_t>(o.max_sweeps, 0x7fffffff);
+    ca.max_outer = (int)std::min<uint64_t>(o.max_outer, 0x7fffffff);
+    ca.tol = o.tol;
+    ca.inner_tol = o.inner_tol;
+    return ca;
+}
+
+cw_status campaign_impl(cw_field* const* F, uint32_t nf, const uint32_t* group,
+                        const int32_t* shapes, uint32_t ngroups, const cw_relax_opts* opts,
+                        cw_relax_stats* stats) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!F || nf == 0 || !group || ngroups == 0 || !stats) return fail(CW_ERR_ARG, "bad argument");
+    Opts o;
+    cw_status st = check_opts(opts, &o);
+    if (st) return st;
+    if (o.crit != CRIT_RESID)
+        return fail(CW_ERR_ARG, "the device-driven batch stops on the relative residual");
+    const bool fp64 = o.precision == CW_PREC_FP64;
+    const int check_every = o.check_every ? o.check_every : 4;
+    for (uint32_t i = 0; i < nf; i++) {
+        if (!F[i]) return fail(CW_ERR_ARG, "null field %u", i);
+        if (F[i]->D > 1) return fail(CW_ERR_ARG, "the device-driven batch solves planes only");
+        if (group[i] >= ngroups) return fail(CW_ERR_ARG, "field %u: group %u of %u", i, group[i], ngroups);
+        for (uint32_t j = 0; j < i; j++)
+            if (F[j] == F[i]) return fail(CW_ERR_ARG, "field %u appears twice", i);
+        if (F[i]->coef_dirty && (st = build_coefficients(F[i]))) return st;
+        if (!fp64 && (st = ensure_fp32(F[i]))) return st;
+        memset(&stats[i], 0, sizeof(cw_relax_stats));
+        stats[i].rel_residual = std::nan("");
+    }
+    // every field's own stream is idle before the graph reads its memory
+    for (uint32_t i = 0; i < nf; i++) CW_HIP(hipStreamSynchronize(F[i]->stream));
+    // Each group's shape; groups of one shape then form one device group (one
+    // tile list and one step kernel a round), since a persistent step kernel
+    // takes every CU and two of them would only run one after the other.
+    std::vector<int> shape_of(ngroups, -1);
+    for (uint32_t g = 0; g < ngroups; g++) {
+        std::vector<cw_field*> members;
+        for (uint32_t i = 0; i < nf; i++)
+            if (group[i] == g) members.push_back(F[i]);
+        if (members.empty()) continue;
+        const int nm = (int)members.size();
+        const int shape = (shapes && shapes[g] >= 0) ? shapes[g]
+                          : fp64 ? default_plain_shape(0, members.data(), nm)
+                                 : default_smooth2d_shape(members.data(), nm);
+        if (shape >= (fp64 ? kNumPlainShapes : kNumSmoothShapes) || !camp_shape_supported(shape, fp64))
+            return fail(CW_ERR_ARG,
+                        "tile shape %d is not one the device-driven batch runs (FP64: 8 or 9; "
+                        "FP32: 15 or 18 to 24)", shape);
+        shape_of[g] = shape;
+    }
+    std::vector<int> distinct;
+    for (uint32_t g = 0; g < ngroups; g++)
+        if (shape_of[g] >= 0 && std::find(distinct.begin(), distinct.end(), shape_of[g]) == distinct.end())
+            distinct.push_back(shape_of[g]);
+    std::vector<std::unique_ptr<CampGroup>> gs;
+    int longest_steps = 1;
+    for (int shape : distinct) {
+        std::vector<cw_field*> members;
+        std::vector<uint32_t> which;
+        for (uint32_t g = 0; g < ngroups; g++) {
+            if (shape_of[g] != shape) continue;
+            for (uint32_t i = 0; i < nf; i++)
+                if (group[i] == g) { members.push_back(F[i]); which.push_back(i); }
+        }
+        auto G = std::make_unique<CampGroup>();
+        Plan& pl = G->pl;
+        G->stream = members[0]->stream;
+        pl.stream = G->stream;
+        pl.fp64 = fp64;
+        pl.plain = true;
+        pl.three_d = false;
+        pl.lower_bound = false;
+        const int nm = (int)members.size();
+        G->shape = pl.shape = shape;
+        G->s = pl.s_main = fp64 ? kPlainShapes[shape].s : kSmoothShapes[shape].s;
+        for (int m = 0; m < nm; m++) {
+            cw_field* f = members[m];
+            cw_relax_stats& S = stats[which[m]];
+            S.sweeps_per_pass = G->s;
+            S.tile_shape = shape;
+            if (!f->has_span) continue;  // nothing free: 0 sweeps, as relax reports it
+            const double omega = pick_omega(o.omega, f);
+            S.omega = omega;
+            S.span_voxels = span_voxels(f);
+            Prob pr;
+            fill_prob_common(pr, f, omega);
+            if (fp64) {
+                CW_HIP(hipMemcpyAsync(f->c[f->cur ^ 1], f->c[f->cur], f->npad * sizeof(double),
+                                      hipMemcpyDeviceToDevice, G->stream));
+                pr.x[0] = f->c[f->cur];
+                pr.x[1] = f->c[f->cur ^ 1];
+                pr.k = f->k;
+                pr.k2 = f->general ? f->k2 : nullptr;
+                pr.c64 = f->c[f->cur];
+                if (f->general) pl.general = true;
+                pl.hp.push_back(pr);
+            } else {
+                Prob p64 = pr;
+                p64.k = f->k;
+                p64.k2 = f->general ? f->k2 : nullptr;
+                p64.c64 = f->c[f->cur];
+                p64.e0 = f->e[0];
+                p64.e1 = f->e[1];
+                p64.r32 = f->r32;
+                p64.lb32 = f->has_sink ? f->lb : nullptr;
+                Prob ps = pr;
+                ps.x[0] = f->e[0];
+                ps.x[1] = f->e[1];
+                ps.k = f->r32;
+                ps.k2 = nullptr;
+                ps.lb = p64.lb32;
+                if (ps.lb) pl.lower_bound = true;
+                pl.hp.push_back(ps);
+                G->hp64.push_back(p64);
+            }
+            Ctl c;
+            init_ctl(c, ST_RUNNING);
+            pl.hc.push_back(c);
+            CampField cf;
+            memset(&cf, 0, sizeof cf);
+            cf.phase = PH_INIT;
+            int longest = 2;
+            for (int a = 0; a < 2; a++) longest = std::max(longest, f->span[2 * a + 1] - f->span[2 * a] + 1);
+            cf.first = std::max(32, longest / 4);
+            cf.hi = std::max(256, 4 * longest);
+            cf.model_rate = model_log_rate(f, omega);
+            G->hst.push_back(cf);
+            G->idx.push_back((int)which[m]);
+            // steps the model problem predicts: its sweeps to tol with a 30%
+            // margin, plus the refinement's residual and correction steps
+            const double need = cf.model_rate < 0.0 ? 1.3 * std::log(o.tol) / cf.model_rate : 1e4;
+            const int steps = (int)std::min(1e7, std::ceil(need / G->s)) + (fp64 ? 2 : 40);
+            longest_steps = std::max(longest_steps, steps);
+            G->pred.push_back(steps);
+        }
+        const int np = (int)G->idx.size();
+        pl.nprob = np;
+        if (np == 0) continue;
+        if ((st = plan_upload(pl))) return st;
+        if (!fp64)
+            for (int p = 0; p < np; p++) {
+                Prob& q = G->hp64[p];
+                const Prob& r = pl.hp[p];
+                memcpy(q.sw_base, r.sw_base, sizeof q.sw_base);
+                memcpy(q.sw_count, r.sw_count, sizeof q.sw_count);
+                memcpy(q.ntx, r.ntx, sizeof q.ntx);
+                memcpy(q.nty, r.nty, sizeof q.nty);
+                q.el_base = r.el_base; q.el_count = r.el_count; q.elx = r.elx; q.ely = r.ely;
+            }
+        const size_t ntiles = pl.tp[TL_MAIN].size();
+        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t b_st = up(np * sizeof(CampField)), b_p64 = up(np * sizeof(Prob)),
+                     b_run = up(4 * sizeof(int)), b_list = up(std::max<size_t>(ntiles, 1) * sizeof(int));
+        CW_HIP(hipMalloc((void**)&G->dev, b_st + b_p64 + b_run + b_list));
+        G->dst = (CampField*)G->dev;
+        G->dp64 = (Prob*)(G->dev + b_st);
+        G->count = (int*)(G->dev + b_st + b_p64);
+        G->list = (int*)(G->dev + b_st + b_p64 + b_run);
+        CW_HIP(hipMemcpyAsync(G->dst, G->hst.data(), np * sizeof(CampField), hipMemcpyHostToDevice,
+                              G->stream));
+        if (!fp64)
+            CW_HIP(hipMemcpyAsync(G->dp64, G->hp64.data(), np * sizeof(Prob), hipMemcpyHostToDevice,
+                                  G->stream));
+        CW_HIP(hipMemsetAsync(G->count, 0, b_run, G->stream));
+        G->gen = fp64 ? pl.general : pl.lower_bound;
+        if ((st = camp_grid(shape, fp64, G->gen, false, (long)ntiles, &G->grid))) return st;
+        if ((st = camp_grid(shape, fp64, G->gen, true, (long)ntiles, &G->grid_tail))) return st;
+        if (fp64) {
+            // the verdict on the input, as the batched FP64 solve takes it:
+            // resid64 and decide fix r2 and b2 (the tolerance's scale) and stop
+            // a field that is already converged
+            const int nt = (int)pl.tp[TL_EL].size();
+            PassArgs a = make_args(pl, TL_EL, DECIDE_RESID, CRIT_RESID, o.tol, 0);
+            hipLaunchKernelGGL(resid64, dim3(nt), dim3(256), 0, pl.stream, a, 0);
+            CW_HIP(hipGetLastError());
+            if ((st = launch_decide(pl, DECIDE_RESID, CRIT_RESID, o.tol, 0))) return st;
+        }
+        gs.push_back(std::move(G));
+    }
+    long steps_run = 0;
+    // the groups as the transition kernels read them, and the shared step base
+    struct Shared {
+        unsigned char* dev = nullptr;
+        ~Shared() { (void)hipFree(dev); }
+    } shared;
+    if (!gs.empty()) {
+        hipStream_t s0 = gs[0]->stream;
+        const size_t b_groups = (gs.size() * sizeof(CampGroupDev) + 255) / 256 * 256;
+        CW_HIP(hipMalloc((void**)&shared.dev, b_groups + 256));
+        CampGroupDev* dgroups = (CampGroupDev*)shared.dev;
+        int* run = (int*)(shared.dev + b_groups);
+        std::vector<CampGroupDev> hg(gs.size());
+        int nfields_all = 0;
+        for (size_t g = 0; g < gs.size(); g++) {
+            gs[g]->run = run;
+            hg[g].ca = camp_args(*gs[g], 0, o, check_every, fp64);
+            hg[g].field_base = nfields_all;
+            hg[g].nfields = (int)gs[g]->idx.size();
+            nfields_all += hg[g].nfields;
+        }
+        CW_HIP(hipMemcpyAsync(dgroups, hg.data(), gs.size() * sizeof(CampGroupDev),
+                              hipMemcpyHostToDevice, s0));
+        CW_HIP(hipMemsetAsync(run, 0, 256, s0));
+        // The set-up each group enqueued on its own stream comes first.
+        std::vector<hipEvent_t> setup(gs.size(), nullptr);
+        for (size_t g = 1; g < gs.size(); g++) {
+            CW_HIP(hipEventCreateWithFlags(&setup[g], hipEventDisableTiming));
+            const hipError_t e1 = hipEventRecord(setup[g], gs[g]->stream);
+            const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(s0, setup[g], 0) : e1;
+            (void)hipEventDestroy(setup[g]);
+            if (e2 != hipSuccess) return fail(CW_ERR_DEVICE, "set-up join failed");
+        }
+        // One graph of kCampSteps rounds: each round the transition for every
+        // group on the first group's stream, then the step launches forked onto
+        // lanes of their own and joined back. A full-width FP64 step is split
+        // over kCampSplit lanes (one queue cannot dispatch a workgroup per tile
+        // fast enough); the FP64 tail form and FP32 take one lane per group.
+        auto capture = [&](bool tail, hipGraphExec_t* out) -> cw_status {
+            struct Lane {
+                int g, part;
+                hipStream_t stream;
+                bool own;
+            };
+            std::vector<Lane> lanes;
+            for (size_t g = 0; g < gs.size(); g++) {
+                const int parts = (fp64 && !tail) ? kCampSplit : 1;
+                for (int q = 0; q < parts; q++) lanes.push_back({(int)g, q, nullptr, false});
+            }
+            std::vector<hipEvent_t> ev(lanes.size(), nullptr);
+            auto drop = [&]() {
+                for (hipEvent_t e : ev)
+                    if (e) (void)hipEventDestroy(e);
+                for (Lane& ln : lanes)
+                    if (ln.own && ln.stream) (void)hipStreamDestroy(ln.stream);
+            };
+            for (size_t i = 0; i < lanes.size(); i++) {
+                Lane& ln = lanes[i];
+                if (i == 0) ln.stream = s0;
+                else if (ln.part == 0) ln.stream = gs[ln.g]->stream;
+                else if (hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking) == hipSuccess) ln.own = true;
+                else { ln.stream = nullptr; drop(); return fail(CW_ERR_DEVICE, "hipStreamCreate failed"); }
+            }
+            for (auto& e : ev)
+                if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                    drop();
+                    return fail(CW_ERR_DEVICE, "hipEventCreate failed");
+                }
+            hipGraph_t graph = nullptr;
+            if (hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                drop();
+                return fail(CW_ERR_DEVICE, "graph capture failed to begin");
+            }
+            cw_status cst = CW_OK;
+            for (int j = 0; j < kCampSteps && cst == CW_OK; j++) {
+                cst = camp_launch_transition(dgroups, (int)gs.size(), nfields_all, j, run, s0);
+                if (cst == CW_OK && lanes.size() > 1 && hipEventRecord(ev[0], s0) != hipSuccess)
+                    cst = fail(CW_ERR_DEVICE, "fork record failed");
+                for (size_t i = 1; i < lanes.size() && cst == CW_OK; i++)
+                    if (hipStreamWaitEvent(lanes[i].stream, ev[0], 0) != hipSuccess)
+                        cst = fail(CW_ERR_DEVICE, "fork wait failed");
+                for (size_t i = 0; i < lanes.size() && cst == CW_OK; i++) {
+                    CampGroup& G = *gs[lanes[i].g];
+                    CampArgs ca = camp_args(G, j, o, check_every, fp64);
+                    const int grid = tail ? G.grid_tail : G.grid;
+                    ca.slot_base = lanes[i].part * grid;
+                    cst = camp_launch_step(G.shape, fp64, G.gen, tail, ca, grid, lanes[i].stream);
+                }
+                for (size_t i = 1; i < lanes.size() && cst == CW_OK; i++)
+                    if (hipEventRecord(ev[i], lanes[i].stream) != hipSuccess ||
+                        hipStreamWaitEvent(s0, ev[i], 0) != hipSuccess)
+                        cst = fail(CW_ERR_DEVICE, "join failed");
+            }
+            if (cst == CW_OK) {
+                hipLaunchKernelGGL(advance_run, dim3(1), dim3(64), 0, s0, run, kCampSteps, 0);
+                if (hipGetLastError() != hipSuccess) cst = fail(CW_ERR_DEVICE, "advance_run failed");
+            }
+            const hipError_t ec = hipStreamEndCapture(s0, &graph);
+            if (cst == CW_OK && ec != hipSuccess)
+                cst = fail(CW_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(ec));
+            if (cst == CW_OK && hipGraphInstantiate(out, graph, nullptr, nullptr, 0) != hipSuccess)
+                cst = fail(CW_ERR_DEVICE, "graph instantiate failed");
+            if (graph) (void)hipGraphDestroy(graph);
+            drop();
+            return cst;
+        };
+        // FP64: full-width rounds while the model problem predicts more tiles
+        // running than twice the tail grid, tail rounds after that; FP32: one form.
+        long head_steps = 0;
+        if (fp64) {
+            std::vector<std::pair<int, long>> ends;  // predicted last step, tiles
+            long tail_grid = 0;
+            for (auto& G : gs) {
+                tail_grid += G->grid_tail;
+                for (size_t p = 0; p < G->idx.size(); p++)
+                    ends.push_back({G->pred[p], (long)G->pl.hp[p].sw_count[TL_MAIN]});
+            }
+            std::sort(ends.begin(), ends.end());
+            long running = 0;
+            for (auto& e : ends) running += e.second;
+            for (auto& e : ends) {
+                if (running < 2 * tail_grid) break;
+                head_steps = e.first;
+                running -= e.second;
+            }
+        }
+        hipGraphExec_t full = nullptr, tailx = nullptr;
+        struct Execs {
+            hipGraphExec_t* a;
+            hipGraphExec_t* b;
+            ~Execs() {
+                if (*a) (void)hipGraphExecDestroy(*a);
+                if (*b) (void)hipGraphExecDestroy(*b);
+            }
+        } execs{&full, &tailx};
+        const long head_chains = fp64 ? (head_steps + kCampSteps - 1) / kCampSteps : 0;
+        if (head_chains > 0 && (st = capture(false, &full))) return st;
+        if ((st = capture(fp64, &tailx))) return st;
+        // launch the predicted number of chains, wait once; a field still
+        // running after that costs one more round trip each time
+        const long limit_steps =
+            (long)(std::min<uint64_t>(o.max_sweeps, 0x7fffffff) + 2 * std::min<uint64_t>(o.max_outer, 1 << 20)) + 8;
+        long chains = std::max(1L, (longest_steps + kCampSteps - 1) / kCampSteps - head_chains), more = 1;
+        for (long c = 0; c < head_chains && st == CW_OK; c++)
+            if (hipGraphLaunch(full, s0) != hipSuccess) st = fail(CW_ERR_DEVICE, "graph launch failed");
+        steps_run += head_chains * kCampSteps;
+        while (st == CW_OK) {
+            for (long c = 0; c < chains && st == CW_OK; c++)
+                if (hipGraphLaunch(tailx, s0) != hipSuccess) st = fail(CW_ERR_DEVICE, "graph launch failed");
+            steps_run += chains * kCampSteps;
+            if (st) break;
+            for (auto& G : gs)
+                if (hipMemcpyAsync(G->hst.data(), G->dst, G->hst.size() * sizeof(CampField),
+                                   hipMemcpyDeviceToHost, s0) != hipSuccess)
+                    st = fail(CW_ERR_DEVICE, "state read failed");
+            if (st == CW_OK && hipStreamSynchronize(s0) != hipSuccess)
+                st = fail(CW_ERR_DEVICE, "graph run failed: %s", hipGetErrorString(hipGetLastError()));
+            if (st) break;
+            bool running = false;
+            for (auto& G : gs)
+                for (const CampField& cf : G->hst) running |= camp_active(cf.phase) || cf.phase == PH_INIT;
+            if (!running || steps_run >= limit_steps) break;
+            chains = more;
+            more = std::min<long>(2 * more, 16);
+        }
+        if (st) return st;
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    cw_status worst = CW_OK;
+    for (auto& G : gs)
+        for (size_t p = 0; p < G->idx.size(); p++) {
+            const CampField& cf = G->hst[p];
+            cw_field* f = F[G->idx[p]];
+            cw_relax_stats& S = stats[G->idx[p]];
+            S.sweeps = (uint64_t)cf.total;
+            S.outer_iters = (uint64_t)cf.outer;
+            S.passes = (uint64_t)steps_run;
+            if (cf.b2 > 0.0) S.rel_residual = std::sqrt(cf.r2 / cf.b2);
+            else if (cf.phase == PH_DONE) S.rel_residual = std::sqrt(cf.r2);
+            S.status = cf.phase == PH_DONE ? CW_OK : cf.nonfinite ? CW_ERR_NONFINITE : CW_ERR_NOT_CONVERGED;
+            if (fp64 && cf.phase == PH_DONE && (cf.parity & 1)) f->cur ^= 1;
+            if (S.status != CW_OK && worst == CW_OK) worst = (cw_status)S.status;
+        }
+    for (uint32_t i = 0; i < nf; i++) stats[i].seconds = secs;
+    if (worst == CW_ERR_NOT_CONVERGED)
+        return fail(CW_ERR_NOT_CONVERGED, "diffusion did not settle within %llu sweeps at tolerance %g",
+                    (unsigned long long)o.max_sweeps, o.tol);
+    if (worst == CW_ERR_NONFINITE) return fail(CW_ERR_NONFINITE, "the residual is not finite");
+    return CW_OK;
+}
+
+cw_status campaign_solve(cw_field* const* F, uint32_t nf, const uint32_t* group,
+                         const int32_t* shapes, uint32_t ngroups, const cw_relax_opts* opts,
+                         cw_relax_stats* stats) {
+    try {
+        return campaign_impl(F, nf, group, shapes, ngroups, opts, stats);
+    } catch (const std::bad_alloc&) {
+        return fail(CW_ERR_DEVICE, "out of host memory");
+    } catch (const std::exception& e) {
+        return fail(CW_ERR_DEVICE, "host error: %s", e.what());
+    }
+}
+
 // Adjoint of the steady state (new; the reference fits parameters by hand).
 // Loss L = 0.5 * sum over all voxels of (c - target)^2. With A c = b on the
 // free voxels and lambda = A^-1 (c - target) (A symmetric, lambda = 0 on held
@@ -1954,6 +2364,12 @@ cw_status cw_field_relax_ex(cw_field* f, const cw_relax_opts* opts, cw_relax_sta
 cw_status cw_batch_relax(cw_field* const* fields, uint32_t count, const cw_relax_opts* opts,
                          cw_relax_stats* stats) {
     return solve(fields, count, opts, stats);
+}
+
+cw_status cw_campaign_relax(cw_field* const* fields, uint32_t count, const uint32_t* group,
+                            const int32_t* shapes, uint32_t ngroups, const cw_relax_opts* opts,
+                            cw_relax_stats* stats) {
+    return campaign_solve(fields, count, group, shapes, ngroups, opts, stats);
 }
 
 cw_status cw_batch_time_passes(cw_field* const* fields, uint32_t count, const cw_relax_opts* opts,

@@ -300,6 +300,27 @@ def batch_relax(fields, check=True, **kw):
     return [s.as_dict() for s in stats], rc
 
 
+def campaign_relax(fields, groups, shapes=None, check=True, **kw):
+    """The device-driven batch of planes (cw_campaign_relax): field i belongs
+    to group groups[i], each group swept with its own tile shape shapes[g]
+    (-1: the library's choice). Returns (per-field stats, status)."""
+    o = make_opts(**kw)
+    n = len(fields)
+    groups = [int(g) for g in groups]
+    assert len(groups) == n
+    ng = max(groups) + 1
+    shapes = [-1] * ng if shapes is None else [int(x) for x in shapes]
+    assert len(shapes) == ng
+    arr = (C.c_void_p * n)(*[f._h.value for f in fields])
+    garr = (C.c_uint32 * n)(*groups)
+    sarr = (C.c_int32 * ng)(*shapes)
+    stats = (RelaxStats * n)()
+    rc = _abi.lib().cw_campaign_relax(arr, n, garr, sarr, ng, C.byref(o), stats)
+    if check:
+        _check(rc)
+    return [s.as_dict() for s in stats], rc
+
+
 def time_passes(fields, passes, **kw) -> float:
     """Average milliseconds of one sweep-kernel launch as cw_batch_relax
     would run it (HIP events on the launching stream). With two to four
@@ -326,6 +347,6 @@ def autotune(fields, launches=10, **kw):
     return [(int(shapes[i]), float(us[i])) for i in range(got.value)]
 
 
-__all__ = ["autotune", "time_passes", "Field", "batch_relax", "make_opts", "CellweaveError", "ConfigError",
+__all__ = ["autotune", "time_passes", "Field", "batch_relax", "campaign_relax", "make_opts", "CellweaveError", "ConfigError",
            "SimulationError", "DeviceError", "CRIT_MAX_CHANGE", "CRIT_REL_RESIDUAL",
            "PREC_FP64", "PREC_FP32_REFINE"]

@@ -146,13 +146,18 @@ def campaign512(count=512, reps=2):
     span = [fc.span_voxels() for fc in flat]
     table = campaign.load_table()
     out = {}
-    for name, prec in (("fp64", PREC_FP64), ("fp32_refine", PREC_FP32_REFINE)):
+    # Both precisions device-driven (cw_campaign_relax: one captured graph,
+    # one host wait); the host-driven FP64 batch (cw_batch_relax per size
+    # class, verdicts read by the host) is timed too, and the speed-ups are
+    # taken against the faster FP64 solve.
+    for name, prec, dd in (("fp64", PREC_FP64, True), ("fp64_host_driven", PREC_FP64, False),
+                           ("fp32_refine", PREC_FP32_REFINE, True)):
         plan = campaign.plan(flat, prec, table)
         times = []
         for rep in range(1 + reps):  # the first solve is the untimed warm-up
             for g, fc in zip(fields, flat):
                 g.fill(fc.initial)
-            stats, secs = campaign.solve(fields, plan, precision=prec)
+            stats, secs = campaign.solve(fields, plan, precision=prec, device_driven=dd)
             if rep:
                 times.append(secs)
         secs = float(np.median(times))
@@ -162,16 +167,20 @@ def campaign512(count=512, reps=2):
                      "total_sweeps": int(sum(st["sweeps"] for st in stats)),
                      "worst_rel_residual": worst,
                      "all_converged": all(st["status"] == 0 for st in stats)}
+    best64 = min((out["fp64"], out["fp64_host_driven"]), key=lambda r: r["seconds"])
     return _line(out["fp32_refine"]["voxel_sweeps_per_s"],
                  {"workload": f"{count} histories x 2 fields (oxygen, signal), edges "
-                              "logU[128,768], batched per size class with graph replays, "
-                              "per-field optimal omega",
+                              "logU[128,768], device-driven (one captured graph for every "
+                              "size class, each with its own tile shape), per-field optimal "
+                              "omega",
                   "fields": len(flat), "voxels": int(sum(fc.n for fc in flat)),
                   "case_build_s": build, "shape_table": {k: table.get(k) for k in ("fp64", "fp32")}},
-                 {"fp64": out["fp64"], "fp32_refine": out["fp32_refine"],
-                  "speedup_vs_fp64": {"throughput": out["fp32_refine"]["voxel_sweeps_per_s"] /
-                                      out["fp64"]["voxel_sweeps_per_s"],
-                                      "time_to_solution": out["fp64"]["seconds"] /
+                 {"fp64": out["fp64"], "fp64_host_driven": out["fp64_host_driven"],
+                  "fp32_refine": out["fp32_refine"],
+                  "speedup_vs_fp64": {"against": "the faster FP64 solve",
+                                      "throughput": out["fp32_refine"]["voxel_sweeps_per_s"] /
+                                      best64["voxel_sweeps_per_s"],
+                                      "time_to_solution": best64["seconds"] /
                                       out["fp32_refine"]["seconds"]}})
 
 

@@ -166,6 +166,26 @@ cw_status cw_field_residual(cw_field *field, double *rel_residual);
 cw_status cw_batch_relax(cw_field *const *fields, uint32_t count, const cw_relax_opts *opts,
                          cw_relax_stats *stats);
 
+/* The campaign solved on the device (new; the histories that
+ * src/campaign.rs:196-204 runs on CPU threads, solved together): planes only,
+ * relative-residual rule, FP64 or FP32 refinement. Field i belongs to group
+ * group[i] < ngroups (a size class), and each group is swept with its own
+ * plane tile shape shapes[g] (-1 or shapes NULL: the library's choice; FP64
+ * takes 8 or 9, FP32 15 or 18 to 24), as cw_autotune ranks them. Every field
+ * keeps its own state on the device: one kernel between two steps moves each
+ * field to its next phase (FP64: sweeps with a residual test every
+ * check_every steps; FP32: FP64 residual, a smoothing segment whose length it
+ * picks from the field's residual history, FP64 correction) and lists the
+ * tiles that work next, and a persistent kernel works them. One captured
+ * graph holds every group's chain of steps; the host launches it as often as
+ * the model problem predicts and reads the fields' states once (once more per
+ * extra launch if a field is still running). FP64 gives the bits and sweeps
+ * of cw_batch_relax at the same shape and check_every. A segment never runs
+ * past max_sweeps. stats has `count` entries. */
+cw_status cw_campaign_relax(cw_field *const *fields, uint32_t count, const uint32_t *group,
+                            const int32_t *shapes, uint32_t ngroups, const cw_relax_opts *opts,
+                            cw_relax_stats *stats);
+
 /* Result of cw_field_adjoint. */
 typedef struct cw_adjoint_result {
     double loss;          /* 0.5 * sum over all voxels of (c - target)^2 */
