@@ -252,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_production(Geo g, const uint32_t* id
     pr[i] = v;
 }
 
-// ---- holds (pde.rs:465-498, coupling.rs:79-81) ----------------------------
+// ---- holds (pde.rs:468-498, coupling.rs:80-82) ----------------------------
 // mode 0: hold where mask != 0; 1: where the lattice has an obstacle;
 // 2: everywhere; 3: release every hold.
 __global__ __launch_bounds__(kBlock) void k_hold(Geo g, int mode, const uint8_t* mask,

@@ -1384,7 +1384,8 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
 // CU loading its tile at the start of each launch together (three 1024^2
 // fields to 1e-8: FP32+refine 14.2 ms against 18.3 ms as one batched launch
 // sequence, FP64 38.2 ms against 57.2 ms). The batch never mixes problems, so
-// each field gets the same bits either way. Larger batches keep the single
+// at one tile shape each field gets the same bits either way (the default
+// shape is picked from the tiles of the whole batch). Larger batches keep the single
 // launch sequence: they fill the GPU on their own. In timing mode each field
 // times its own launches and the result is the longest average interval.
 constexpr uint32_t kConcurrentMax = 4;  // hardware queues of a process

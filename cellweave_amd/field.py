@@ -188,12 +188,12 @@ class Field:
         _check(_abi.lib().cw_field_apply_background_uptake(self._h, lattice._h, rate))
 
     def apply_production(self, lattice: Lattice, per_cell):
-        """coupling.rs:60-73."""
+        """coupling.rs:61-73."""
         r = np.ascontiguousarray(per_cell, dtype=np.float64).reshape(-1)
         _check(_abi.lib().cw_field_apply_production(self._h, lattice._h, _dptr(r), r.size))
 
     def hold_obstacles_at(self, lattice: Lattice, value: float):
-        """coupling.rs:79-81."""
+        """coupling.rs:80-82."""
         _check(_abi.lib().cw_field_hold_obstacles_at(self._h, lattice._h, value))
 
     def hold_where(self, mask, value: float):
@@ -209,15 +209,15 @@ class Field:
                                                 value))
 
     def hold_medium_bath(self, lattice: Lattice, value: float):
-        """coupling.rs:148-150."""
+        """coupling.rs:156-158."""
         _check(_abi.lib().cw_field_hold_medium_bath(self._h, lattice._h, value))
 
     def release_all(self):
-        """pde.rs:465-467."""
+        """pde.rs:468-470."""
         _check(_abi.lib().cw_field_release_all(self._h))
 
     def free_count(self) -> int:
-        """pde.rs:458-461."""
+        """pde.rs:460-463."""
         v = C.c_uint64()
         _check(_abi.lib().cw_field_free_count(self._h, C.byref(v)))
         return int(v.value)
@@ -288,8 +288,10 @@ class Field:
 
 def batch_relax(fields, check=True, **kw):
     """Solve every field; returns per-field stats. Two to four fields run as
-    concurrent solves on their own streams, more in one launch sequence;
-    each field gets the same bits either way."""
+    concurrent solves on their own streams, more in one launch sequence.
+    Under the max-change rule each field gets the bits it gets alone; under
+    the relative-residual rule and in FP32 refinement, the bits it gets alone
+    at the same tile shape (the default shape counts the whole batch's tiles)."""
     o = make_opts(**kw)
     n = len(fields)
     arr = (C.c_void_p * n)(*[f._h.value for f in fields])

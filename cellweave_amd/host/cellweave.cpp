@@ -352,6 +352,22 @@ void hold_obstacles_at(ScalarField& field, Lattice& lattice, double value) {
     field.touched_on_device({ScalarField::kValues, ScalarField::kHeld});
 }
 
+void hold_vessels_at(ScalarField& field, Lattice& lattice, double value,
+                     const std::function<bool(uint32_t, uint32_t)>& carries) {
+    same_shape(field, lattice);
+    const auto [w, h, d] = lattice.dims3();
+    (void)d;
+    const auto& grid = lattice.grid();
+    std::vector<uint8_t> mask(grid.size(), 0);
+    for (size_t i = 0; i < grid.size(); i++) {
+        // position(i), cpm.rs:960-963
+        const uint32_t x = (uint32_t)(i % w), y = (uint32_t)((i / w) % h);
+        mask[i] = grid[i] == Lattice::OBSTACLE && carries(x, y) ? 1 : 0;
+    }
+    check(cw_field_hold_where(field.device(), mask.data(), value));
+    field.touched_on_device({ScalarField::kValues, ScalarField::kHeld});
+}
+
 void hold_medium_bath(ScalarField& field, Lattice& lattice, double value) {
     same_shape(field, lattice);
     check(cw_field_hold_medium_bath(field.device(), lattice.device(), value));
@@ -364,6 +380,29 @@ std::vector<double> mean_per_cell(ScalarField& field, Lattice& lattice) {
     check(cw_field_mean_per_cell(field.device(), lattice.device(), (uint32_t)out.size(),
                                  out.data()));
     return out;
+}
+
+double mean_in(const ScalarField& field, const Lattice& lattice, uint32_t id) {
+    same_shape(field, lattice);
+    const auto& grid = lattice.grid();
+    const auto& v = field.values();
+    double sum = 0.0;
+    uint32_t count = 0;
+    for (size_t i = 0; i < grid.size(); i++)
+        if (grid[i] == id) {
+            sum = sum + v[i];
+            count += 1;
+        }
+    return count == 0 ? 0.0 : sum / (double)count;
+}
+
+double still_medium_level(const ScalarField& oxygen, double diffusion) {
+    const double pi = 3.14159265358979323846;
+    const double taken = oxygen.inflow();
+    const double radius = std::cbrt(3.0 * (double)oxygen.free_count() / (4.0 * pi));
+    if (!(radius > 0.0)) return 1.0;
+    const double level = 1.0 - taken / (4.0 * pi * diffusion * radius);
+    return level > 0.0 ? level : 0.0;
 }
 
 }  // namespace cellweave

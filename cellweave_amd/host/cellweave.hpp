@@ -125,8 +125,8 @@ class ScalarField {
     cw_relax_stats relax_with(const cw_relax_opts& opts);
 
     double inflow() const;            // pde.rs:415-456
-    size_t free_count() const;        // pde.rs:458-461
-    void release_all();               // pde.rs:465-467
+    size_t free_count() const;        // pde.rs:460-463
+    void release_all();               // pde.rs:468-470
     std::optional<Span> free_span() const;  // pde.rs:305-320
 
     void hold_pixels(double value, const std::function<bool(uint32_t, uint32_t)>& is_source);
@@ -153,6 +153,8 @@ class ScalarField {
     friend void apply_background_uptake(ScalarField&, Lattice&, double);
     friend void apply_production(ScalarField&, Lattice&, const std::vector<double>&);
     friend void hold_obstacles_at(ScalarField&, Lattice&, double);
+    friend void hold_vessels_at(ScalarField&, Lattice&, double,
+                                const std::function<bool(uint32_t, uint32_t)>&);
     friend void hold_medium_bath(ScalarField&, Lattice&, double);
     friend std::vector<double> mean_per_cell(ScalarField&, Lattice&);
 
@@ -178,9 +180,22 @@ class ScalarField {
 void apply_uptake(ScalarField& field, Lattice& lattice, double rate);             // :26-39
 void apply_background_uptake(ScalarField& field, Lattice& lattice, double rate);  // :47-53
 void apply_production(ScalarField& field, Lattice& lattice,
-                      const std::vector<double>& per_cell);                      // :60-73
-void hold_obstacles_at(ScalarField& field, Lattice& lattice, double value);       // :79-81
-void hold_medium_bath(ScalarField& field, Lattice& lattice, double value);        // :148-150
+                      const std::vector<double>& per_cell);                      // :61-73
+void hold_obstacles_at(ScalarField& field, Lattice& lattice, double value);       // :80-82
+// :89-99: every obstacle voxel whose (x, y) `carries` says carries blood is held at value
+void hold_vessels_at(ScalarField& field, Lattice& lattice, double value,
+                     const std::function<bool(uint32_t, uint32_t)>& carries);
+void hold_medium_bath(ScalarField& field, Lattice& lattice, double value);        // :156-158
 std::vector<double> mean_per_cell(ScalarField& field, Lattice& lattice);          // :164-182
+// :187-201: the mean over the voxels whose occupant is `id` (any id, the
+// medium and obstacles included), added in increasing voxel order; 0 for none
+double mean_in(const ScalarField& field, const Lattice& lattice, uint32_t id);
+
+// crates/engine/src/simulation.rs:2114-2125 (settle_bath_level): the level a
+// still medium of diffusion coefficient `diffusion`, held far away at the
+// source's level 1 and taking up nothing, sits at around the tissue: a ball of
+// the free voxels' volume that takes up what the held voxels hand the field,
+// 1 - inflow / (4 pi diffusion R), floored at 0; 1 when nothing is free.
+double still_medium_level(const ScalarField& oxygen, double diffusion);
 
 }  // namespace cellweave

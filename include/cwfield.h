@@ -159,8 +159,13 @@ cw_status cw_field_residual(cw_field *field, double *rel_residual);
  * Five or more fields: one launch sequence, every field swept by the same
  * kernel launches. Two to four fields on distinct streams: concurrent solves,
  * one internal thread and the field's own stream each. Either way each field
- * stops on its own criterion and gets the bits it gets when solved alone.
- * stats (may be NULL) has `count` entries. Returns CW_ERR_NOT_CONVERGED if
+ * stops on its own criterion. Under CW_CRIT_MAX_CHANGE each field gets the
+ * bits it gets when solved alone (an exact stop replays a launch's tail);
+ * under CW_CRIT_REL_RESIDUAL and in FP32 refinement it gets the bits it gets
+ * alone at the same tile_shape: with tile_shape -1 the library picks the
+ * shape from the tiles of the whole batch, and the shape sets the sweeps
+ * between residual tests and the segment lengths. stats (may be NULL) has
+ * `count` entries. Returns CW_ERR_NOT_CONVERGED if
  * any field did not converge. The fields must not be in use elsewhere during
  * the call. */
 cw_status cw_batch_relax(cw_field *const *fields, uint32_t count, const cw_relax_opts *opts,
@@ -261,12 +266,12 @@ cw_status cw_field_apply_uptake_per_cell(cw_field *field, cw_lattice *lattice,
 /* coupling.rs:47-53 apply_background_uptake: every medium voxel (id 0) takes
  * up max(rate, 0); other voxels keep their sink. */
 cw_status cw_field_apply_background_uptake(cw_field *field, cw_lattice *lattice, double rate);
-/* coupling.rs:60-73 apply_production: clear the sources, then every voxel of
+/* coupling.rs:61-73 apply_production: clear the sources, then every voxel of
  * a cell id < ncells secretes per_cell[id] where that is positive. */
 cw_status cw_field_apply_production(cw_field *field, cw_lattice *lattice, const double *per_cell,
                                     uint32_t ncells);
 
-/* coupling.rs:79-81 hold_obstacles_at: hold every obstacle voxel at value. */
+/* coupling.rs:80-82 hold_obstacles_at: hold every obstacle voxel at value. */
 cw_status cw_field_hold_obstacles_at(cw_field *field, cw_lattice *lattice, double value);
 /* pde.rs:491-498 hold_where with the predicate as a byte mask (nonzero =
  * hold): held, and the value set. */
@@ -277,12 +282,12 @@ cw_status cw_field_hold_where(cw_field *field, const uint8_t *mask, double value
  * everything outside that box and everything reached is held, and then every
  * held voxel takes `value`. */
 cw_status cw_field_hold_bath_at(cw_field *field, const uint8_t *open, double value);
-/* coupling.rs:148-150 hold_medium_bath: hold_bath_at with open = medium (id 0). */
+/* coupling.rs:156-158 hold_medium_bath: hold_bath_at with open = medium (id 0). */
 cw_status cw_field_hold_medium_bath(cw_field *field, cw_lattice *lattice, double value);
-/* pde.rs:465-467 release_all. */
+/* pde.rs:468-470 release_all. */
 cw_status cw_field_release_all(cw_field *field);
 
-/* pde.rs:458-461 free_count: voxels not held. */
+/* pde.rs:460-463 free_count: voxels not held. */
 cw_status cw_field_free_count(cw_field *field, uint64_t *count);
 /* pde.rs:415-456 inflow: the flux D (c_j - c_i) over every face between a
  * free voxel i and a held voxel j. Summed in a fixed tree (the same bits on

@@ -425,6 +425,105 @@ TEST(gathers_skip_medium_obstacles_and_unknown_ids) {  // coupling.rs:26-73
     CHECK(f.value_at(3) == 0.7 && f.held()[3] == 1);
 }
 
+TEST(a_vessel_feeds_the_cell_beside_it_and_not_the_one_far_away) {  // coupling.rs:281-302
+    // an obstacle disc of radius 4 at (40, 40), cells of radius 5 at (50, 40)
+    // (id 1) and (70, 40) (id 2)
+    std::vector<uint32_t> g(80 * 80, 0);
+    auto disc = [](int x, int y, int cx, int cy, int r) {
+        return (x - cx) * (x - cx) + (y - cy) * (y - cy) <= r * r;
+    };
+    for (int y = 0; y < 80; y++)
+        for (int x = 0; x < 80; x++) {
+            uint32_t& v = g[(size_t)y * 80 + x];
+            if (disc(x, y, 40, 40, 4)) v = Lattice::OBSTACLE;
+            else if (disc(x, y, 50, 40, 5)) v = 1;
+            else if (disc(x, y, 70, 40, 5)) v = 2;
+        }
+    Lattice l(80, 80);
+    l.set_grid(g);
+    l.set_cells({0, 1, 1});
+    auto field = ScalarField::create(80, 80, D, 0.0, 0.0);
+    apply_uptake(field, l, 0.003);
+    hold_obstacles_at(field, l, 1.0);
+    field.relax(TimeStep(1.0), 1e-7, 50000);
+    const auto o2 = mean_per_cell(field, l);
+    CHECK(o2[1] > o2[2] + 0.1);
+    CHECK(o2.size() == l.cell_count() + 1);
+    // the same through hold_vessels_at with every vessel carrying blood
+    // (coupling.rs:84-88: then it is hold_obstacles_at)
+    auto again = ScalarField::create(80, 80, D, 0.0, 0.0);
+    apply_uptake(again, l, 0.003);
+    hold_vessels_at(again, l, 1.0, [](uint32_t, uint32_t) { return true; });
+    again.relax(TimeStep(1.0), 1e-7, 50000);
+    CHECK(mean_per_cell(again, l) == o2);
+    CHECK(mean_in(again, l, 1) == o2[1] && mean_in(again, l, 2) == o2[2]);
+}
+
+TEST(hold_vessels_at_holds_only_the_obstacles_that_carry) {  // coupling.rs:89-99
+    Lattice l(5, 2);
+    const uint32_t O = Lattice::OBSTACLE;
+    l.set_grid({O, O, 0, 1, O, 0, O, O, 2, 0});
+    l.set_cells({0, 1, 1});
+    auto f = ScalarField::create(5, 2, D, 0.0, 0.0);
+    hold_vessels_at(f, l, 0.8, [](uint32_t x, uint32_t y) { return x < 2 || y == 1; });
+    const std::vector<uint8_t> want = {1, 1, 0, 0, 0, 0, 1, 1, 0, 0};
+    CHECK(f.held() == want);
+    for (size_t i = 0; i < 10; i++) CHECK(f.value_at(i) == (want[i] ? 0.8 : 0.0));
+}
+
+TEST(mean_in_is_the_mean_over_one_id_in_voxel_order) {  // coupling.rs:187-201
+    Lattice l(4, 2);
+    const uint32_t O = Lattice::OBSTACLE;
+    l.set_grid({1, 0, 1, O, 2, 1, 0, 1});
+    l.set_cells({0, 1, 1});
+    auto f = ScalarField::create(4, 2, D, 0.0, 0.0);
+    const double v[8] = {0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8};
+    f.hold_where(0.0, [](size_t) { return false; });
+    for (size_t i = 0; i < 8; i++) f.hold_where(v[i], [i](size_t j) { return j == i; });
+    CHECK(mean_in(f, l, 1) == (((0.1 + 0.3) + 0.6) + 0.8) / 4.0);
+    CHECK(mean_in(f, l, 2) == 0.5);
+    CHECK(mean_in(f, l, 0) == (0.2 + 0.7) / 2.0);  // the medium too, unlike mean_per_cell
+    CHECK(mean_in(f, l, O) == 0.4);
+    CHECK(mean_in(f, l, 7) == 0.0);  // an id with no voxel
+}
+
+TEST(a_still_medium_sits_below_the_source_by_what_the_tissue_takes_up) {  // simulation.rs:2657-2689
+    // The first cell of the volume scenario (simulation.rs:2619-2637): a ball
+    // of 123 voxels at the centre of a 40^3 box, each taking up 0.01, in a
+    // medium held as a bath at 1; settled as Simulation::start settles it
+    // (warm-up tolerance 1e-8, simulation.rs:95-98).
+    Lattice l(40, 40, 40);
+    std::vector<uint32_t> g(40 * 40 * 40, 0);
+    size_t ball = 0;
+    for (int z = 0; z < 40; z++)
+        for (int y = 0; y < 40; y++)
+            for (int x = 0; x < 40; x++)
+                if ((x - 20) * (x - 20) + (y - 20) * (y - 20) + (z - 20) * (z - 20) <= 9) {
+                    g[((size_t)z * 40 + y) * 40 + x] = 1;
+                    ball++;
+                }
+    CHECK(ball == 123);
+    l.set_grid(g);
+    l.set_cells({0, 1});
+    auto o2 = ScalarField::with_depth(40, 40, 40, D, 0.0, 1.0);
+    apply_uptake(o2, l, 0.01);
+    o2.release_all();
+    hold_medium_bath(o2, l, 1.0);
+    o2.relax(TimeStep(1.0), 1e-8, 2000000);
+    const double pi = 3.14159265358979323846;
+    const double radius = std::cbrt(3.0 * 123.0 / (4.0 * pi));
+    const double expected = 1.0 - 0.01 * 123.0 / (4.0 * pi * 0.2 * radius);
+    const double level = still_medium_level(o2, 0.2);
+    CHECK(std::fabs(level - expected) < 1e-6);
+    // a medium oxygen goes through faster is depleted less
+    const double faster = still_medium_level(o2, 2.0);
+    CHECK(faster > level && faster < 1.0);
+    // nothing free: the level is the source's
+    auto held = ScalarField::with_depth(4, 4, 4, D, 0.0, 1.0);
+    held.hold_where(1.0, [](size_t) { return true; });
+    CHECK(still_medium_level(held, 0.2) == 1.0);
+}
+
 TEST(host_and_device_copies_stay_in_step) {
     // a setter after a device gather, a gather after a setter, a hold after a step
     Lattice l(16, 16);

@@ -1,6 +1,8 @@
 """The reference's own ScalarField and coupling tests (crates/engine/src/pde.rs
 tests module, crates/engine/src/coupling.rs tests module), run on the CPU
 oracle. Names follow the reference's test functions."""
+import math
+
 import numpy as np
 import pytest
 
@@ -280,3 +282,97 @@ def test_gathers_skip_medium_obstacles_and_unknown_ids():  # coupling.rs:26-73
     assert list(f.production) == [0, 0, 3.0, 0, 0, 3.0]
     f.apply_uptake_per_cell(grid, np.array([1, 1, 1], np.uint8), np.array([0.0, 0.1, 0.2]))
     assert list(f.uptake) == [0, 0.1, 0.2, 0, 0, 0.2]
+
+
+def _random_lattice(seed, w=41, h=29, ncells=37):
+    """Ids over 0..ncells+4 (some past the table), obstacles, and medium."""
+    rng = np.random.default_rng(seed)
+    grid = rng.integers(0, ncells + 5, w * h).astype(np.uint32)
+    grid[rng.random(w * h) < 0.1] = OBSTACLE
+    grid[rng.random(w * h) < 0.15] = 0
+    return grid, rng
+
+
+def test_mean_per_cell_against_an_independent_sum():  # coupling.rs:164-182
+    """Per id below the table: the sum of the values of its voxels, in
+    increasing voxel order, over their count; zero for the medium, for ids
+    with no voxel, and nothing for obstacles or ids past the table. A
+    plausible slip (count + 1, the medium or obstacles counted, ids past the
+    table wrapped) moves at least one entry."""
+    for seed in range(4):
+        grid, rng = _random_lattice(seed)
+        ncells = 37
+        f = OracleField(41, 29, 1, 0.2, 0.0, 0.0)
+        f.values[:] = rng.uniform(0.0, 2.0, grid.size)
+        got = f.mean_per_cell(grid, ncells)
+        want = np.zeros(ncells)
+        for k in range(1, ncells):
+            idx = np.nonzero(grid == k)[0]
+            s = 0.0
+            for i in idx:  # voxel order, as the reference adds them
+                s = s + float(f.values[i])
+            want[k] = s / idx.size if idx.size else 0.0
+        assert np.array_equal(got, want)
+        # and against numpy's own reduction, to rounding
+        ok = (grid > 0) & (grid < ncells)
+        cnt = np.bincount(grid[ok], minlength=ncells)
+        tot = np.bincount(grid[ok], weights=f.values[ok], minlength=ncells)
+        ref = np.divide(tot, cnt, out=np.zeros(ncells), where=cnt > 0)
+        ref[0] = 0.0
+        assert np.allclose(got, ref, rtol=1e-14, atol=0.0)
+
+
+def test_per_cell_gathers_against_numpy():  # coupling.rs:26-73
+    """apply_uptake_per_cell and apply_production against a numpy statement:
+    a voxel of a cell id below the table takes its cell's rate where that is
+    positive (and, for uptake, where the cell's kind consumes); the medium,
+    obstacles and ids past the table take nothing."""
+    for seed in range(4):
+        grid, rng = _random_lattice(100 + seed)
+        ncells = 37
+        rates = rng.uniform(-1e-3, 2e-3, ncells)
+        rates[3] = float("nan")
+        consumes = (rng.random(ncells) < 0.7).astype(np.uint8)
+        consumes[0] = 0
+        f = OracleField(41, 29, 1, 0.2, 0.0, 0.0)
+        f.apply_uptake_per_cell(grid, consumes, rates)
+        ok = (grid > 0) & (grid < ncells)
+        ids = np.where(ok, grid, 0).astype(np.int64)
+        r = np.where(ok, rates[ids], 0.0)
+        want_up = np.where(ok & (consumes[ids] != 0) & (r > 0), r, 0.0)
+        assert np.array_equal(f.uptake, want_up)
+        f.apply_production(grid, rates)
+        want_pr = np.where(ok & (r > 0), r, 0.0)
+        assert np.array_equal(f.production, want_pr)
+
+
+def still_medium_level(field, diffusion):
+    """simulation.rs:2114-2125 settle_bath_level: 1 - inflow / (4 pi D R), R
+    the radius of a ball of the free voxels' volume, floored at 0; 1 when
+    nothing is free."""
+    radius = (3.0 * field.free_count() / (4.0 * math.pi)) ** (1.0 / 3.0)
+    if not radius > 0.0:
+        return 1.0
+    return max(1.0 - field.inflow() / (4.0 * math.pi * diffusion * radius), 0.0)
+
+
+def test_a_still_medium_sits_below_the_source_by_what_the_tissue_takes_up():
+    """simulation.rs:2657-2689: the first cell of the volume scenario
+    (simulation.rs:2619-2637), a ball of 123 voxels at the centre of a 40^3
+    box each taking up 0.01 in a medium held as a bath at 1, settled as
+    Simulation::start settles it (1e-8); the level is the closed form within
+    1e-6, and a medium oxygen crosses faster is depleted less."""
+    n = 40
+    zz, yy, xx = np.mgrid[0:n, 0:n, 0:n]
+    ball = (((xx - 20) ** 2 + (yy - 20) ** 2 + (zz - 20) ** 2) <= 9).reshape(-1)
+    assert ball.sum() == 123
+    f = OracleField(n, n, n, 0.2, 0.0, 1.0)
+    f.uptake[:] = np.where(ball, 0.01, 0.0)
+    f.hold_bath_at(1.0, (~ball).astype(np.uint8))
+    f.relax(1e-8, 2_000_000, threads=4)
+    radius = (3.0 * 123.0 / (4.0 * math.pi)) ** (1.0 / 3.0)
+    expected = 1.0 - 0.01 * 123.0 / (4.0 * math.pi * 0.2 * radius)
+    level = still_medium_level(f, 0.2)
+    assert abs(level - expected) < 1e-6, (level, expected)
+    faster = still_medium_level(f, 2.0)
+    assert level < faster < 1.0
