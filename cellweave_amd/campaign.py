@@ -81,15 +81,18 @@ def plan(cases_, precision=PREC_FP32_REFINE, table=None):
 
 
 def solve(fields, plan_, precision=PREC_FP32_REFINE, tol=1e-8, use_graph=1,
-          max_sweeps=2_000_000, omega=-1.0, concurrent=True, device_driven=True):
+          max_sweeps=2_000_000, omega=-1.0, concurrent=False, device_driven=True):
     """Solve every field to `tol`. omega < 0 gives each field its own
     2/(1+sin(pi/L)), L its longest free-span edge. Returns (stats per field,
     seconds).
 
     device_driven (the default): one cw_campaign_relax call for every class,
-    each with its plan's tile shape. Otherwise one cw_batch_relax launch
-    sequence per size class, verdicts read by the host; with `concurrent` the
-    classes run at once, one host thread each."""
+    each with its plan's tile shape, from this thread. Otherwise one
+    cw_batch_relax launch sequence per size class, verdicts read by the host,
+    the classes one after another; with `concurrent` they run at once, one
+    host thread each (about 2% faster; a profiled run of that form once ended
+    in a host segmentation fault whose cause was not found, so it is not the
+    default)."""
     stats = [None] * len(fields)
     if device_driven:
         groups = [0] * len(fields)

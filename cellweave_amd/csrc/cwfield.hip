@@ -1315,6 +1315,14 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     // stopped.
     std::vector<char> act(np, 1);
     int nact = np;
+    // Sweeps field p may still run under max_sweeps. In a batch the launches
+    // run as many sweeps as the longest segment needs (s at a time), and a
+    // field's own budget is judged only between launches, so its allowance
+    // is whole launches; alone, a remainder runs one sweep a launch.
+    auto room = [&](int p) -> uint64_t {
+        const uint64_t left = o.max_sweeps > (uint64_t)total[p] ? o.max_sweeps - (uint64_t)total[p] : 0;
+        return np > 1 ? left / (uint64_t)s * (uint64_t)s : left;
+    };
     for (uint64_t it = 0; it < o.max_outer && any_status(outer, ST_RUNNING); it++) {
         int nrun = 0;
         for (int p = 0; p < np; p++) nrun += outer.hc[p].status == ST_RUNNING;
@@ -1326,13 +1334,12 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         }
         int most = 0;
         for (int p = 0; p < np; p++) {
-            const bool run = outer.hc[p].status == ST_RUNNING;
+            const bool run = outer.hc[p].status == ST_RUNNING && room(p) > 0;
             init_ctl(inner.hc[p], run ? ST_RUNNING : ST_IDLE);
             if (!run) continue;
             int k = next_segment(hist[p], hist_at[p], o.tol, o.inner_tol, first, 16, hi, s,
                                  model_rate[p]);
-            if ((uint64_t)total[p] + k > o.max_sweeps)
-                k = (int)(o.max_sweeps - (uint64_t)total[p]);
+            if ((uint64_t)k > room(p)) k = (int)room(p);
             inner.hc[p].budget = std::max(k, 1);
             most = std::max(most, inner.hc[p].budget);
             outer_it[p]++;
@@ -1359,7 +1366,7 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
             if (inner.hc[p].status == ST_IDLE) continue;
             hist[p].push_back(c.b2 > 0.0 ? std::sqrt(c.r2 / c.b2) : std::sqrt(c.r2));
             hist_at[p].push_back(total[p]);
-            if (c.status == ST_RUNNING && (uint64_t)total[p] < o.max_sweeps) any_left = true;
+            if (c.status == ST_RUNNING && room(p) > 0) any_left = true;
         }
         if (!any_left) break;
     }

@@ -81,6 +81,21 @@ def test_fp32_budget_not_a_multiple_of_the_depth(shape):
     assert st[0]["status"] != 0 and st[0]["sweeps"] == 37
 
 
+def test_fp32_batch_budget_is_a_hard_limit():
+    """A batch of five fields of different sizes (one launch sequence, the
+    launches as long as the longest segment) under a max_sweeps that is not
+    a multiple of the launch depth: no field runs past it."""
+    say("case fp32 batch budget 101")
+    hist = cases.campaign_histories(count=3, lo=96, hi=320, seed=4500)
+    flat = [fc for h in hist for fc in h][:5]
+    gs = [gpu_from_case(fc) for fc in flat]
+    st, _ = batch_relax(gs, check=False, omega=-1.0, criterion=CRIT_REL_RESIDUAL, tolerance=1e-30,
+                        max_sweeps=101, precision=PREC_FP32_REFINE)
+    for s in st:
+        assert s["status"] != 0 and s["sweeps"] <= 101, s
+        assert s["sweeps"] > 101 - s["sweeps_per_pass"], s
+
+
 @pytest.mark.parametrize("shape", PLAIN_SHAPES)
 def test_general_form_every_shape(shape):
     say(f"case general form plain shape={shape}")
