@@ -212,6 +212,8 @@ def main():
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--spp", type=int, default=0, help="FP32 sweeps per pass (0 = library)")
     ap.add_argument("--spp64", type=int, default=0)
+    ap.add_argument("--shape32", type=int, default=-1,
+                    help="FP32 smoother tile shape (-1 = library); overrides --spp")
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--time-launches", type=int, default=200)
@@ -263,13 +265,18 @@ def main():
         for g, fc in zip(fields, fcs):
             g.fill(fc.initial)
 
+    def shape_kw(precision):
+        if precision == PREC_FP32_REFINE and args.shape32 >= 0:
+            return {"tile_shape": args.shape32, "sweeps_per_pass": 0}
+        return {"sweeps_per_pass": args.spp if precision == PREC_FP32_REFINE else args.spp64}
+
     def solve(precision, spp):
         reset()
         sync()
         t0 = time.perf_counter()
         stats, _ = batch_relax(fields, criterion=CRIT_REL_RESIDUAL, tolerance=tol,
                                max_sweeps=2_000_000, omega=omega, precision=precision,
-                               sweeps_per_pass=spp, inner_tolerance=args.inner_tol)
+                               inner_tolerance=args.inner_tol, **shape_kw(precision))
         sync()
         return time.perf_counter() - t0, stats
 
@@ -314,13 +321,16 @@ def main():
 
     # Live kernel timing of the dominant kernel (the FP32 smoother launch).
     spp_used = st32[0]["sweeps_per_pass"]
+    # waves and rows a wave of the register-blocked FP32 shapes (S, NW, NR)
+    nw32, nr32 = {15: (16, 8), 18: (16, 8), 19: (16, 8), 20: (16, 8), 21: (8, 8), 22: (8, 8),
+                  23: (8, 8), 24: (16, 8)}.get(st32[0]["tile_shape"], ("?", "?"))
     ms32 = time_passes(fields, args.time_launches, criterion=CRIT_REL_RESIDUAL, tolerance=tol,
-                       omega=omega, precision=PREC_FP32_REFINE, sweeps_per_pass=args.spp)
+                       omega=omega, precision=PREC_FP32_REFINE, **shape_kw(PREC_FP32_REFINE))
     bytes32 = sum(s * bytes_per_voxel_pass(fc, "fp32") for s, fc in zip(span, fcs))
     achieved = bytes32 / (ms32 * 1e-3) / 1e9
     reset()
     ms64 = time_passes(fields, args.time_launches, criterion=CRIT_REL_RESIDUAL, tolerance=tol,
-                       omega=omega, precision=PREC_FP64, sweeps_per_pass=args.spp64)
+                       omega=omega, precision=PREC_FP64, **shape_kw(PREC_FP64))
     spp64_used = st64[0]["sweeps_per_pass"]
     bytes64 = sum(s * 24.0 for s in span)
     achieved64 = bytes64 / (ms64 * 1e-3) / 1e9
@@ -355,6 +365,15 @@ def main():
         solve_bytes += s_vox * (b_pass * st["sweeps"] / st["sweeps_per_pass"] +
                                 24.0 * (st["outer_iters"] + 1) + 20.0 * st["outer_iters"])
     solve_gbs = solve_bytes / (ms_step * 1e-3) / 1e9
+    # The smoother's per-launch working set: every field's pass bytes at once
+    # (the three launches run concurrently). Within the 256 MB Infinity Cache
+    # the sweeps are VALU-bound (SQ_INSTS_VALU at 0.84 of the roof at 1024^2);
+    # beyond it each launch streams from HBM as well.
+    working_set = sum(s_vox * bytes_per_voxel_pass(fc, "fp32") for s_vox, fc in zip(span, fcs))
+    binding = ("VALU (FP32 register-blocked smoother; working set %.0f MB, inside the 256 MB "
+               "Infinity Cache)" % (working_set / 1e6) if working_set <= 256e6 else
+               "HBM and VALU (FP32 register-blocked smoother; working set %.0f MB, beyond the "
+               "256 MB Infinity Cache)" % (working_set / 1e6))
 
     out = {
         "metric": "voxel-sweeps/s & % HBM roofline, FP32+refine vs FP64 SOR at 1e-8 residual",
@@ -387,9 +406,9 @@ def main():
                      "voxel-sweep, FP64 residual 24 B and correction 20 B a voxel per outer "
                      "iteration) over ms_per_step",
             "alg_bytes_per_solve": solve_bytes,
-            "binding_resource": "VALU (FP32 register-blocked smoother, MALL-resident working set)",
+            "binding_resource": binding,
             "kernel": {
-                "name": f"relax2d_reg<float,{spp_used},16,8,...> (FP32 smoother, {spp_used} sweeps "
+                "name": f"relax2d_reg<float,{spp_used},{nw32},{nr32},...> (FP32 smoother, {spp_used} sweeps "
                         f"a launch; one launch per field, the {len(fcs)} fields' launches "
                         "concurrent on their own streams)",
                 "launch_ms": ms32, "alg_bytes_all_fields_per_launch": bytes32,

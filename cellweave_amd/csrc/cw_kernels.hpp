@@ -99,6 +99,8 @@ struct PassArgs {
     int project;           // resid64: project onto the floor (0 for the adjoint's linear system)
     int pad_;
     const int* run;        // graph replays: device base (pass, sweeps) added to pass, sweeps_before
+    const double* sums;    // non-null: the previous (checked) launch's residual sums per problem,
+                           // folded by fold_sums; null: each workgroup sums the tile partials
 };
 
 // A launch replayed from a captured graph carries pass and sweeps_before as
@@ -155,18 +157,55 @@ __device__ __forceinline__ unsigned long long change_bits(double m) {
 }
 
 // Sum of one problem's tile partials of one ring slot, by one whole wave:
-// lanes stride over the tiles, a fixed shuffle tree adds, so every wave that
-// computes it gets the same bits.
+// lane l adds tiles l, l + 64, l + 128, ... in that order, then a fixed
+// shuffle tree adds the lanes, so every wave that computes it gets the same
+// bits. Each (r2, b2) pair is one 16-byte load, eight of them in flight a
+// lane: a problem's partials come from other XCDs' writes, and a wave that
+// waited on one load at a time would spend about a microsecond per 64 tiles.
 __device__ __forceinline__ void problem_sums(const double* part, int base, int cnt, double* r2,
                                              double* b2) {
     const int lane = threadIdx.x & 63;
+    const double2* p2 = reinterpret_cast<const double2*>(part) + base;
     double r = 0.0, b = 0.0;
-    for (int t = lane; t < cnt; t += 64) {
-        r += part[2 * (base + t)];
-        b += part[2 * (base + t) + 1];
+    int t = lane;
+    for (; t + 7 * 64 < cnt; t += 8 * 64) {
+        double2 v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) v[k] = p2[t + 64 * k];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            r += v[k].x;
+            b += v[k].y;
+        }
+    }
+    for (; t < cnt; t += 64) {
+        const double2 v = p2[t];
+        r += v.x;
+        b += v.y;
     }
     *r2 = wave_sum(r);
     *b2 = wave_sum(b);
+}
+
+// Second level of the stopping verdict's reduction, launched right after a
+// checked launch on a plan whose problems have many tiles: one wave per
+// problem sums that launch's tile partials once (problem_sums: the order, and
+// so the bits, every workgroup would get) and writes (r2, b2) to sums[2p].
+// The next launch's workgroups then read two doubles each instead of all of
+// their problem's partials, which would be O(tiles^2) reads a launch.
+static __global__ __launch_bounds__(256) void fold_sums(PassArgs a_in) {
+    const PassArgs a = resolve(a_in);
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= a.nprob) return;  // wave-uniform
+    const Prob& pr = a.probs[p];
+    double r2, b2;
+    problem_sums(a.partials + (a.pass & 1) * a.part_stride, pr.sw_base[a.tl], pr.sw_count[a.tl],
+                 &r2, &b2);
+    if ((threadIdx.x & 63) == 0) {
+        double* out = const_cast<double*>(a.sums);
+        out[2 * p] = r2;
+        out[2 * p + 1] = b2;
+    }
 }
 
 struct Verdict {
@@ -234,9 +273,14 @@ __device__ inline bool sweep_prologue(const PassArgs& a, const Prob& pr, int p, 
     const bool have_r2 = a.prev_check && a.crit == CRIT_RESID && a.pass > 0;
     double r2 = 0.0, b2 = 0.0;
     if (have_r2 && threadIdx.x < 64) {
-        const int tl = a.prev_tl;
-        problem_sums(a.partials + ((a.pass - 1) & 1) * a.part_stride, pr.sw_base[tl],
-                     pr.sw_count[tl], &r2, &b2);
+        if (a.sums) {
+            r2 = a.sums[2 * p];
+            b2 = a.sums[2 * p + 1];
+        } else {
+            const int tl = a.prev_tl;
+            problem_sums(a.partials + ((a.pass - 1) & 1) * a.part_stride, pr.sw_base[tl],
+                         pr.sw_count[tl], &r2, &b2);
+        }
     }
     if (threadIdx.x == 0) {
         int run = 0;

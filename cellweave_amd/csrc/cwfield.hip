@@ -105,6 +105,19 @@ constexpr bool two_sweep(int shape3) { return k3DShapes[shape3].kind == 2; }
 constexpr int kNum3DShapes = (int)(sizeof(k3DShapes) / sizeof(k3DShapes[0]));
 constexpr int kDefault3DShape = 1;
 
+// Planes whose largest problem has at least this many sweep tiles fold each
+// checked launch's tile partials once per problem (fold_sums) instead of in
+// every workgroup of the next launch. Below it the extra launch costs more
+// than the workgroups' reads (at the default shapes a 1024^2 plane has 170
+// to 220 tiles, 4096^2 2,700 to 3,200). CW_FOLD_TILES overrides it; both ways
+// give the same bits.
+constexpr int kFoldTiles = 512;
+int fold_threshold() {
+    const char* e = getenv("CW_FOLD_TILES");
+    if (!e || !*e) return kFoldTiles;
+    const long v = strtol(e, nullptr, 10);
+    return v < 1 ? 1 : (int)std::min<long>(v, 1L << 30);
+}
 
 // Volumes: the z-streaming kernels are latency-bound when workgroups are
 // few, so the halo planes a shorter chunk re-reads matter less than the
@@ -307,7 +320,13 @@ struct Plan {
     int* dtp[3] = {nullptr, nullptr, nullptr};
     double* partials = nullptr;
     int part_stride = 0;
-    size_t cap_prob = 0, cap_ctl = 0, cap_tp[3] = {0, 0, 0}, cap_part = 0;
+    // Per-problem residual sums of a checked launch (fold_sums), used on
+    // sweep tilings whose largest problem has at least fold_tiles tiles.
+    double* sums = nullptr;
+    int max_tiles[3] = {0, 0, 0};
+    int fold_tiles = kFoldTiles;
+    bool fold(int tl) const { return !three_d && max_tiles[tl] >= fold_tiles; }
+    size_t cap_prob = 0, cap_ctl = 0, cap_tp[3] = {0, 0, 0}, cap_part = 0, cap_sums = 0;
     // host mirror of the pass sequence of the current run
     int pass = 0, sweeps_before = 0, prev_s = 0, prev_tl = 0, prev_check = 0;
     // captured chunk of passes (hipGraph), replayed with a device-resident base
@@ -332,11 +351,12 @@ struct Plan {
         Layout L;
         memset(&L, 0, sizeof L);
         L.buf[0] = dp; L.buf[1] = dc; L.buf[2] = dtp[0]; L.buf[3] = dtp[1];
-        L.buf[4] = dtp[2]; L.buf[5] = partials; L.buf[6] = run;
+        L.buf[4] = dtp[2]; L.buf[5] = partials; L.buf[6] = run; L.buf[7] = sums;
         L.n[0] = (int)tp[0].size(); L.n[1] = (int)tp[1].size(); L.n[2] = (int)tp[2].size();
         L.n[3] = nprob; L.n[4] = part_stride; L.n[5] = s_main;
         L.n[6] = shape | (shape3 << 8) | (three_d << 16) | (fp64 << 17) | (plain << 18) |
                  (general << 19) | (lower_bound << 20);
+        L.n[7] = fold(TL_MAIN) | (fold(TL_ONE) << 1);
         return L;
     }
     Layout g_layout{};
@@ -373,7 +393,7 @@ struct Plan {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (arena) return;  // nothing here was allocated by the plan
         (void)hipFree(run);
-        (void)hipFree(dp); (void)hipFree(dc); (void)hipFree(partials);
+        (void)hipFree(dp); (void)hipFree(dc); (void)hipFree(partials); (void)hipFree(sums);
         for (int i = 0; i < 3; i++) (void)hipFree(dtp[i]);
     }
 };
@@ -453,6 +473,12 @@ cw_status plan_upload(Plan& pl, const std::vector<char>* active = nullptr) {
     size_t maxt = 1;
     for (int tl = 0; tl < 3; tl++) maxt = std::max(maxt, pl.tp[tl].size());
     pl.part_stride = (int)(2 * maxt);
+    for (int tl = 0; tl < 3; tl++) {
+        pl.max_tiles[tl] = 0;
+        if (tl == TL_EL) continue;
+        for (int p = 0; p < np; p++) pl.max_tiles[tl] = std::max(pl.max_tiles[tl], pl.hp[p].sw_count[tl]);
+    }
+    pl.fold_tiles = fold_threshold();
     if (pl.arena) {
         // problems, control blocks, the three tile tables and the partials,
         // each on a 256-byte boundary
@@ -466,6 +492,7 @@ cw_status plan_upload(Plan& pl, const std::vector<char>* active = nullptr) {
         pl.dc = (Ctl*)take(std::max(1, np) * sizeof(Ctl));
         for (int tl = 0; tl < 3; tl++) pl.dtp[tl] = (int*)take(std::max<size_t>(1, pl.tp[tl].size()) * sizeof(int));
         pl.partials = (double*)take(2 * (size_t)pl.part_stride * sizeof(double));
+        pl.sums = (double*)take(2 * (size_t)std::max(1, np) * sizeof(double));
         if (off > pl.arena_bytes)
             return fail(CW_ERR_ARG,
                         "the solve needs %zu bytes of launch-plan scratch; the allocation-free "
@@ -478,6 +505,7 @@ cw_status plan_upload(Plan& pl, const std::vector<char>* active = nullptr) {
             if (!pl.tp[tl].empty() && (st = grow(&pl.dtp[tl], &pl.cap_tp[tl], pl.tp[tl].size())))
                 return st;
         if ((st = grow(&pl.partials, &pl.cap_part, 2 * (size_t)pl.part_stride))) return st;
+        if ((st = grow(&pl.sums, &pl.cap_sums, 2 * (size_t)std::max(1, np)))) return st;
     }
     for (int tl = 0; tl < 3; tl++) {
         if (pl.tp[tl].empty()) continue;
@@ -512,9 +540,11 @@ PassArgs make_args(Plan& pl, int tl, int mode, int crit, double tol, int max_swe
     return a;
 }
 
-cw_status launch_sweep(Plan& pl, const PassArgs& a) {
-    const int nt = (int)pl.tp[a.tl].size();
+cw_status launch_sweep(Plan& pl, const PassArgs& a_in) {
+    const int nt = (int)pl.tp[a_in.tl].size();
     if (nt == 0) return CW_OK;
+    PassArgs a = a_in;
+    a.sums = a.prev_check && pl.fold(a.prev_tl) ? pl.sums : nullptr;
     const int shape = a.tl == TL_MAIN ? pl.shape : 0;
     const dim3 grid(nt), b256(256), b512(512), b1024(1024);
     hipStream_t st = pl.stream;
@@ -591,6 +621,11 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a) {
 #undef CW_LB
 #undef CW_L
     CW_HIP(hipGetLastError());
+    if (a.check && pl.fold(a.tl)) {
+        a.sums = pl.sums;
+        hipLaunchKernelGGL(fold_sums, dim3((pl.nprob + 3) / 4), dim3(256), 0, st, a);
+        CW_HIP(hipGetLastError());
+    }
     return CW_OK;
 }
 
