@@ -161,6 +161,7 @@ def tune(cases_, per_class=0, launches=10, confirm=4, tol=1e-8):
                     best = secs if best is None else min(best, secs)
                 timed.append([shape, 1e3 * best])
             timed.sort(key=lambda e: e[1])
+            print(f"tune {key} {cls}: {len(fields)} fields, solve ms by shape {timed}", flush=True)
             table[key][cls] = timed[0][0]
             table["ranking"][f"{key} {cls}"] = runnable[:6]
             table["confirmed_ms"][f"{key} {cls}"] = timed
