@@ -148,14 +148,16 @@ def _field(name, w, h, d, diff, decay, initial, hold, uptake, production, grid=N
                      seed=seed)
 
 
-def oxygen_disc_200(seed: int = 1000):
+def oxygen_disc_200(seed: int = 1000, n: int = 200):
     """200^2 plane, one oxygen field, ~450 Voronoi cells in a radius-60 disc,
-    per-cell uptake U[0.5, 1.5]e-3 /s, ring held at 1. The solve uses omega 1.9."""
+    per-cell uptake U[0.5, 1.5]e-3 /s, ring held at 1. The solve uses omega 1.9.
+    Another n scales the disc with the edge and the cell count with the area."""
     rng = np.random.default_rng(seed)
-    n = 200
-    grid = voronoi_disc(n, n, 100.0, 100.0, 60.0, 450, rng)
-    rates = np.zeros(451)
-    rates[1:] = rng.uniform(0.5e-3, 1.5e-3, 450)
+    scale = n / 200.0
+    ncells = int(round(450 * scale * scale))
+    grid = voronoi_disc(n, n, n / 2.0, n / 2.0, 60.0 * scale, ncells, rng)
+    rates = np.zeros(ncells + 1)
+    rates[1:] = rng.uniform(0.5e-3, 1.5e-3, ncells)
     up = gather(grid, rates)
     return [_field("oxygen", n, n, 1, d_px(2000.0), 0.0, 1.0, 1.0, up, np.zeros(n * n),
                    grid, rates, seed)]
