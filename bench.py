@@ -99,7 +99,8 @@ def host_threads():
     return max(1, min(n, cap))
 
 
-def cpu_baseline(fc, sweeps_to_tol=None, omega=None, samples=5, sample_s=1.5, threads=None):
+def cpu_baseline(fc, sweeps_to_tol=None, omega=None, samples=5, sample_s=1.5, threads=None,
+                 tol_label="1e-8"):
     """The oracle (C restatement of pde.rs: FP64, read-old/write-new
     buffers, OpenMP over rows in chunks of 4) on this host, on the oxygen
     field of the bench workload at the bench's omega:
@@ -140,11 +141,12 @@ def cpu_baseline(fc, sweeps_to_tol=None, omega=None, samples=5, sample_s=1.5, th
                 out.append(span * n / dt)
         return float(sorted(out)[len(out) // 2]), out
 
+    dims = f"{fc.w}x{fc.h}" + (f"x{fc.d}" if fc.d > 1 else "")
     all_rate, all_samples = rate(threads)
     one_rate, one_samples = rate(1)
     res = {"value": all_rate, "unit": "voxel-sweeps/s", "cores": threads, "kind": "port",
            "cpu": cpu_model(), "nproc": os.cpu_count(),
-           "sample": f"FP64 red-black sweeps of the {fc.w}x{fc.h} oxygen field ({span} swept "
+           "sample": f"FP64 red-black sweeps of the {dims} {fc.name} field ({span} swept "
                      f"voxels), median of {samples} samples of ~{sample_s:.0f} s after a "
                      f"warm-up, CPU restatement of pde.rs (the Rust binary cannot be built here)",
            "samples": all_samples,
@@ -156,11 +158,73 @@ def cpu_baseline(fc, sweeps_to_tol=None, omega=None, samples=5, sample_s=1.5, th
         for _ in range(sweeps_to_tol):
             f.sor_sweep(omega, threads, sp)
         dt = time.perf_counter() - t0
-        res["time_to_1e-8"] = {"seconds": dt, "sweeps": sweeps_to_tol, "cores": threads,
-                               "rel_residual": f.rel_residual()[0],
-                               "field": "oxygen (the GPU FP64 solve of it takes "
-                                        "fp64.ms_per_solve for all three fields)"}
+        res["time_to_" + tol_label] = {"seconds": dt, "sweeps": sweeps_to_tol, "cores": threads,
+                                       "rel_residual": f.rel_residual()[0],
+                                       "field": f"{fc.name}, {fc.w}x{fc.h}x{fc.d}"}
     return res
+
+
+def cpu_campaign(flat, sweeps, picks=8, threads=None):
+    """The oracle on a sample of the campaign: `picks` histories spread over
+    the size range (sorted by edge, evenly spaced), each field swept the
+    number of FP64 sweeps its GPU solve took to 1e-8, on all host threads;
+    the whole campaign's CPU time is estimated from the sample's voxel-sweep
+    rate and the campaign's total FP64 voxel-sweeps."""
+    from cellweave_amd.campaign import omega_for
+    from oracle.orf import OracleField
+
+    threads = host_threads() if threads is None else threads
+    nh = len(flat) // 2
+    order = sorted(range(nh), key=lambda h: flat[2 * h].w)
+    hs = [order[round(i * (nh - 1) / max(1, picks - 1))] for i in range(picks)]
+    vs, secs = 0, 0.0
+    for h in hs:
+        for k in (2 * h, 2 * h + 1):
+            fc = flat[k]
+            f = OracleField(fc.w, fc.h, fc.d, fc.diff, fc.decay, fc.initial)
+            f.values[:] = fc.initial_values()
+            f.copy_next_from_values()
+            f.uptake[:] = fc.uptake
+            f.production[:] = fc.production
+            f.held[:] = fc.held
+            sp = f.free_span()
+            omega = omega_for(fc)  # the per-field omega of the GPU solve
+            t0 = time.perf_counter()
+            for _ in range(sweeps[k]):
+                f.sor_sweep(omega, threads, sp)
+            secs += time.perf_counter() - t0
+            vs += fc.span_voxels() * sweeps[k]
+    total = sum(fc.span_voxels() * s for fc, s in zip(flat, sweeps))
+    rate = vs / secs
+    return {"value": rate, "unit": "voxel-sweeps/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(), "nproc": os.cpu_count(),
+            "sample": f"{picks} histories ({2 * picks} fields, edges "
+                      f"{min(flat[2 * h].w for h in hs)}..{max(flat[2 * h].w for h in hs)}) "
+                      "swept their GPU FP64 sweep counts to 1e-8, CPU restatement of pde.rs",
+            "sample_seconds": secs,
+            "campaign_seconds_estimate": total / rate}
+
+
+def cpu_for_workload(name, line):
+    """The CPU restatement beside a workload line, on its representative
+    field (the campaign: a sample of histories)."""
+    from cellweave_amd import cases
+
+    if name == "oxygen200":
+        (fc,) = cases.oxygen_disc_200()
+        return cpu_baseline(fc, sweeps_to_tol=line["fp64"]["sweeps"][0], omega=1.9)
+    if name == "spheroid256":
+        (fc,) = cases.spheroid()
+        return cpu_baseline(fc, sweeps_to_tol=line["fp64"]["sweeps"][0],
+                            omega=line["config"]["omega"], samples=3)
+    if name == "adjoint192":
+        fc, _ = cases.adjoint_spheroid()
+        return cpu_baseline(fc, sweeps_to_tol=line["forward"]["fp64_sweeps"], omega=line["omega"],
+                            samples=3, tol_label="1e-13")
+    if name == "campaign512":
+        hist = cases.campaign_histories(count=512)
+        return cpu_campaign([fc for h in hist for fc in h], line["_fp64_sweeps"])
+    return None
 
 
 def cpu_tissue_steps(gpu_sweeps, steps=4, threads=None):
@@ -233,9 +297,13 @@ def main():
         from cellweave_amd import workloads
 
         line = workloads.WORKLOADS[args.workload]()
-        if args.workload == "tissue120" and not args.no_cpu:
+        if not args.no_cpu:
             log("cpu baseline")
-            line["cpu_baseline"] = cpu_tissue_steps(line["sweeps_per_step"])
+            if args.workload == "tissue120":
+                line["cpu_baseline"] = cpu_tissue_steps(line["sweeps_per_step"])
+            else:
+                line["cpu_baseline"] = cpu_for_workload(args.workload, line)
+        line.pop("_fp64_sweeps", None)
         print(json.dumps(line))
         return
 

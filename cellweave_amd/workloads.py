@@ -145,7 +145,7 @@ def campaign512(count=512, reps=2):
     fields = campaign.make_fields(flat)
     span = [fc.span_voxels() for fc in flat]
     table = campaign.load_table()
-    out = {}
+    out, per_field = {}, {}
     # Both precisions device-driven (cw_campaign_relax: one captured graph,
     # one host wait); the host-driven FP64 batch (cw_batch_relax per size
     # class, verdicts read by the host) is timed too, and the speed-ups are
@@ -161,6 +161,7 @@ def campaign512(count=512, reps=2):
             if rep:
                 times.append(secs)
         secs = float(np.median(times))
+        per_field[name] = [int(st["sweeps"]) for st in stats]
         vs = sum(s * st["sweeps"] for s, st in zip(span, stats))
         worst = max(st["rel_residual"] for st in stats)
         out[name] = {"seconds": secs, "seconds_each": times, "voxel_sweeps_per_s": vs / secs,
@@ -181,7 +182,9 @@ def campaign512(count=512, reps=2):
                                       "throughput": out["fp32_refine"]["voxel_sweeps_per_s"] /
                                       best64["voxel_sweeps_per_s"],
                                       "time_to_solution": best64["seconds"] /
-                                      out["fp32_refine"]["seconds"]}})
+                                      out["fp32_refine"]["seconds"]},
+                  # per-field FP64 sweep counts, for a CPU timing of the same solves
+                  "_fp64_sweeps": per_field["fp64"]})
 
 
 def neighbour_defect(c, w, h, d):
