@@ -863,6 +863,11 @@ cw_status build_coefficients(cw_field* f, bool may_alloc) {
                        (const double*)f->up, (const double*)f->pr, (int)general, f->k, f->k2);
     CW_HIP(hipGetLastError());
     if (f->r32) CW_HIP(hipMemsetAsync(f->r32, 0xFF, f->npad * sizeof(float), f->stream));
+    // The FP32 smoother needs a zero correction on every held cell; outside
+    // the new free span no pass writes it, so clear what an earlier, larger
+    // span may have left there.
+    for (int b = 0; b < 2; b++)
+        if (f->e[b]) CW_HIP(hipMemsetAsync(f->e[b], 0, f->npad * sizeof(float), f->stream));
     CW_HIP(hipStreamSynchronize(f->stream));
     f->general = general;
     f->has_sink = general ? sc[2] != 0 : sc[1] != 0;

@@ -670,12 +670,14 @@ __global__ __launch_bounds__(NW * 64) void smooth2d(PassArgs a_in) {
 // Neighbour values across lanes without LDS: DPP wave shifts (GFX9 family).
 // from_left gives lane l the value of lane l-1, from_right that of lane l+1;
 // the wave's end lanes get 0 (their cells sit on the region's edge and are
-// never updated).
+// never updated). The FP32 forms ask for the zero through the bound control,
+// so a shift the compiler cannot fold into its add needs no zeroed
+// destination first.
 __device__ __forceinline__ float from_left(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float from_right(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xF, 0xF, true));
 }
 __device__ __forceinline__ double from_left(double v) {
     const long long b = __double_as_longlong(v);
@@ -859,8 +861,18 @@ struct RegCells {
     T kc[2][NR];   // PLAIN: coefficient (NaN = held or outside); smoother: beta
     T k2[2][NR];   // PLAIN+GEN: uptake; smoother: lower bound
     T al[2][NR];   // smoother: alpha = omega d / A (0 on held or outside cells)
-    T ga[2][NR];   // smoother: gamma = -omega (0 on held or outside cells)
+    T ga[2][NR];   // FP64 smoother: gamma = -omega (0 on held or outside cells); the
+                   // FP32 smoother uses -omega everywhere (below)
 };
+
+// The FP32 smoother keeps gamma = -omega on every cell: a held or outside
+// cell has alpha = beta = 0 and a correction of exactly zero (the correction
+// buffers are zero wherever no free cell's value was stored: zeroed when
+// made, outside the free span whenever the coefficients are rebuilt, and on
+// every held cell of the span by the residual pass), so
+// 0 + 0*sum + fma(-omega, 0, 0) keeps it at +0 with one register array fewer.
+template <typename T, bool PLAIN>
+constexpr bool kConstGamma = !PLAIN && std::is_same<T, float>::value;
 
 template <typename T, bool PLAIN, bool GEN, int NR, int C, int P0, int S>
 __device__ __forceinline__ void reg_half(RegCells<T, NR>& R, T above, T below, int r0, int lo,
@@ -873,6 +885,20 @@ __device__ __forceinline__ void reg_half(RegCells<T, NR>& R, T above, T below, i
         const int q = P0 ^ (i & 1);  // compile-time after unrolling
         const int r = r0 + i;
         if (PLAIN && (r < lo || r > hiY)) continue;
+        if constexpr (kConstGamma<T, PLAIN>) {
+            // FP32 smoother, cells by column parity: this colour's cell of
+            // row i is at parity q, the other colour's at q ^ 1, and the
+            // cells above and below (the other colour) at parity q too.
+            const T o = R.st[q ^ 1][i];
+            const T across = q ? from_right(o) : from_left(o);
+            const T up = i > 0 ? R.st[q][i - 1] : above;
+            const T dn = i < NR - 1 ? R.st[q][i + 1] : below;
+            const T before = R.st[q][i];
+            const T sum = (across + o) + (up + dn);
+            const T after = before + fma_t(R.al[q][i], sum, fma_t(-omega, before, R.kc[q][i]));
+            R.st[q][i] = GEN ? max_t(after, R.k2[q][i]) : after;
+            continue;
+        }
         const int x = 2 * lane + q;
         const T o = R.st[C ^ 1][i];
         const T across = q ? from_right(o) : from_left(o);
@@ -907,10 +933,11 @@ __device__ __forceinline__ void reg_half(RegCells<T, NR>& R, T above, T below, i
             }
             R.st[C][i] = upd ? after : before;
         } else {
-            // before + alpha*sum + (beta - omega*before): a held or outside
-            // cell has alpha = beta = gamma = 0 and keeps its value exactly.
-            // Cells of the stale halo are updated too: they are invalid at
-            // this depth whatever they hold, and never reach a valid cell.
+            // FP64 smoother: before + alpha*sum + (beta - omega*before); a
+            // held or outside cell has alpha = beta = gamma = 0 and keeps its
+            // value exactly. Cells of the stale halo are updated too: they
+            // are invalid at this depth whatever they hold, and never reach
+            // a valid cell.
             const T sum = (across + o) + (up + dn);
             after = before + fma_t(R.al[C][i], sum, fma_t(R.ga[C][i], before, R.kc[C][i]));
             // lower bound (GEN: the launch has one; -inf for a field without)
