@@ -177,25 +177,28 @@ int default_volume_shape(cw_field* const* F, uint32_t nf, bool fp64) {
 }
 int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 : 15; }
 
-// FP32 planes with no sweep count asked for: each relax2d_reg workgroup
-// (one per CU, 96-119 VGPRs a lane) loads its tile, about 14 us at full
-// occupancy, then sweeps it S times at about 0.9 us a sweep, and the launch
-// lasts as many of those rounds as it takes to run every tile. A deeper tile
-// spreads the load over more sweeps but leaves a smaller interior, so more
-// tiles; pick the S (8, 10, 12 or 16) with the least time per sweep over
-// whole rounds.
-//
-// Very small batches take 128 x 64 regions at S = 12 (512-thread workgroups)
-// when all their tiles fit in a fifth of the CUs: more, shorter workgroups
-// finish a launch sooner than a few 128-row ones. FP32+refine to 1e-8, ms
-// (128 x 64 at S = 12 / 128 x 128 at S = 12 / 128 x 128 at S = 16):
-//   128^2 0.87 / 0.98 / 0.89    200^2 4.18 / 4.76 / 4.26    256^2 1.67 / 1.78 / 1.61
-//   320^2 2.07 / 2.16 / 1.96    384^2 2.64 / 2.62 / 2.44    512^2 - / 3.99 / 3.59
-//   1024^2 - / 7.28 / 6.73; three 256^2 fields 2.36 / 2.19 / 1.98; two 512^2
-//   5.36 / 4.16 / 3.66; three 1024^2 - / 12.2 / 14.2 (768 tiles at S = 16)
+// FP32 planes with no sweep count asked for, from the tiles of the whole
+// batch (peers included):
+//   - 128 x 64 regions at S = 12 (shape 23, 512-thread workgroups, two a CU)
+//     when they fit on the CUs at once: more, shorter workgroups;
+//   - else 128 x 128 regions at S = 16 (shape 24) when they fit at once: one
+//     round of the deepest tile, whose per-sweep latency is least;
+//   - else the S (8, 10, 12 or 16 on 128 x 128) with the least tile-work a
+//     sweep, tiles(S) x (L + S w) / S with a tile load L = 20 sweeps w: with
+//     many tiles the launches of several rounds and fields overlap, so work,
+//     not rounds, sets the time.
+// FP32+refine to 1e-8, ms (shapes 23 / 24 / 15 / 19 / 20, the rule's choice
+// in brackets):
+//   128^2 0.74 / 0.78 / 0.96 / 0.94 / 0.87 (23)   256^2 1.10 / 1.37 / 1.82 / 1.59 / 1.39 (23)
+//   512^2 2.14 / 2.52 / 3.33 / 3.00 / 2.77 (23)   768^2 3.36 / 3.47 / 4.35 / 3.91 / 3.62 (23)
+//   1024^2 7.04 / 4.29 / 5.87 / 5.07 / 4.73 (24)  2048^2 37.1 / 27.9 / 22.8 / 26.9 / 24.5 (15)
+//   three 256^2 1.68 / 1.73 / 2.17 / 2.04 / 1.93 (23)   two 512^2 2.60 / 2.81 / 3.70 / 3.28 / 3.06 (23)
+//   three 512^2 3.05 / 2.88 / 3.80 / 3.40 / 3.11 (24)   three 1024^2 15.9 / 11.3 / 8.48 / 8.93 / 9.41 (15)
+//   three 2048^2 117 / 84.8 / 59.8 / 60.5 / 65.7 (15)
 constexpr int kPlaneDeepShapes[] = {15, 19, 20, 24};  // S = 8, 10, 12, 16 on 128 x 128 regions
 constexpr int kPlaneShortShape = 23;                  // S = 12 on 128 x 64 regions
-constexpr double kTileLoadUs = 14.0, kTileSweepUs = 0.9;
+constexpr int kPlaneDeepest = 24;
+constexpr double kTileLoadSweeps = 20.0;
 long plane_tiles(cw_field* const* F, uint32_t nf, const Shape& sh) {
     const int tx = 128 - 4 * sh.s, ty = sh.ry - 4 * sh.s;
     long tiles = 0;
@@ -213,17 +216,16 @@ int default_smooth2d_shape(cw_field* const* F, uint32_t nf) {
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         cus <= 0)
         cus = 256;
-    if (plane_tiles(F, nf, kSmoothShapes[kPlaneShortShape]) <= cus / 5) return kPlaneShortShape;
+    if (plane_tiles(F, nf, kSmoothShapes[kPlaneShortShape]) <= 2L * cus) return kPlaneShortShape;
+    if (plane_tiles(F, nf, kSmoothShapes[kPlaneDeepest]) <= cus) return kPlaneDeepest;
     int best = kPlaneDeepShapes[0];
-    double best_us = 0.0;
+    double best_work = 0.0;
     for (int si : kPlaneDeepShapes) {
         const Shape sh = kSmoothShapes[si];
-        const long tiles = plane_tiles(F, nf, sh);
-        const long rounds = std::max(1L, (tiles + cus - 1) / cus);
-        const double us = rounds * (kTileLoadUs + sh.s * kTileSweepUs) / sh.s;
-        if (si == kPlaneDeepShapes[0] || us < best_us) {
+        const double work = (double)plane_tiles(F, nf, sh) * (kTileLoadSweeps + sh.s) / sh.s;
+        if (si == kPlaneDeepShapes[0] || work < best_work) {
             best = si;
-            best_us = us;
+            best_work = work;
         }
     }
     return best;

@@ -115,12 +115,12 @@ def test_small_campaign_converges_and_matches_single_solves(precision):
             assert bits_equal(g.download(), fields[k].download())
 
 
-def test_plane_smoothing_depth_follows_workgroup_rounds():
-    """With no sweep count asked for, FP32 planes take the temporal blocking
-    depth with the least time per sweep over whole rounds of one workgroup
-    per CU: 12 for three 1024^2 fields (507 tiles, two rounds either way),
-    8 for one free 2048^2 field (484 tiles fill two rounds, deeper tiles
-    need three). MI355X has 256 CUs."""
+def test_plane_smoothing_depth_for_many_tiles():
+    """With no sweep count asked for and more tiles than fit on the CUs at
+    once, FP32 planes take the depth with the least tile-work a sweep,
+    tiles(S) x (20 + S) / S: 8 sweeps for three 1024^2 fields (363 tiles,
+    against 432, 507 and 768 at 10, 12 and 16) and for one free 2048^2 field
+    (484 tiles, against 576, 676 and 1,024). MI355X has 256 CUs."""
     from cellweave_amd.field import Field
 
     say("case smoothing depth 3 x 1024^2")
@@ -128,8 +128,8 @@ def test_plane_smoothing_depth_follows_workgroup_rounds():
     gs = [gpu_from_case(fc) for fc in fcs]
     st, _ = batch_relax(gs, criterion=CRIT_REL_RESIDUAL, tolerance=1e-3, omega=1.99,
                         precision=PREC_FP32_REFINE)
-    assert [s["sweeps_per_pass"] for s in st] == [12, 12, 12]
-    assert [s["tile_shape"] for s in st] == [20, 20, 20]  # 128-row regions
+    assert [s["sweeps_per_pass"] for s in st] == [8, 8, 8]
+    assert [s["tile_shape"] for s in st] == [15, 15, 15]  # 128-row regions
     say("case smoothing depth 2048^2")
     g = Field(2048, 2048, 1, 0.3, 0.05, 0.0)
     g.set_source(2e-3)
@@ -139,12 +139,13 @@ def test_plane_smoothing_depth_follows_workgroup_rounds():
     assert np.max(np.abs(g.download() - 2e-3 / 0.05)) <= 1e-5 * (2e-3 / 0.05)
 
 
-def test_small_planes_take_short_regions():
-    """A plane batch whose 64-row, 12-sweep tiles all fit in a fifth of the
-    CUs takes them (shape 23); larger small batches take 16 sweeps on 128-row
-    regions (shape 24) while those fit in one round. Fields solved
-    concurrently count together. The smoother's bits do not depend on the
-    region shape."""
+def test_default_fp32_depth_follows_the_batch():
+    """With no sweep count asked for, an FP32 plane batch takes 64-row regions
+    at 12 sweeps (shape 23) while they fit on the CUs twice over (two
+    512-thread workgroups a CU), else 16 sweeps on 128-row regions (shape 24)
+    while those fit at once, else the depth with the least tile-work a sweep
+    (8 sweeps, shape 15, for three 1024^2 fields). Fields solved concurrently
+    count together. The smoother's bits do not depend on the region shape."""
     say("case short regions 200^2 oxygen")
     (fc,) = cases.oxygen_disc_200()
     g = gpu_from_case(fc)
@@ -157,16 +158,16 @@ def test_small_planes_take_short_regions():
                      precision=PREC_FP32_REFINE, tile_shape=20)
     assert s20["sweeps"] == s["sweeps"]
     assert np.array_equal(g.download(), v_short)
-    say("case 16 sweeps 3 x 256^2 concurrent")
-    fcs = cases.multifield_plane(n=256)
-    gs = [gpu_from_case(fc) for fc in fcs]
-    st, _ = batch_relax(gs, criterion=CRIT_REL_RESIDUAL, tolerance=1e-3, omega=1.9,
-                        precision=PREC_FP32_REFINE)
-    assert [x["tile_shape"] for x in st] == [24, 24, 24]
-    assert [x["sweeps_per_pass"] for x in st] == [16, 16, 16]
-    st1, _ = batch_relax(gs[:1], criterion=CRIT_REL_RESIDUAL, tolerance=1e-3, omega=1.9,
-                         precision=PREC_FP32_REFINE)
-    assert st1[0]["tile_shape"] == 24
+    for n, want_all, want_one in ((256, 23, 23), (512, 24, 23), (1024, 15, 24)):
+        say(f"case default depth 3 x {n}^2 concurrent and one alone")
+        fcs = cases.multifield_plane(n=n)
+        gs = [gpu_from_case(fc) for fc in fcs]
+        st, _ = batch_relax(gs, criterion=CRIT_REL_RESIDUAL, tolerance=1e-3, omega=1.9,
+                            precision=PREC_FP32_REFINE)
+        assert [x["tile_shape"] for x in st] == [want_all] * 3, (n, st)
+        st1, _ = batch_relax(gs[:1], criterion=CRIT_REL_RESIDUAL, tolerance=1e-3, omega=1.9,
+                             precision=PREC_FP32_REFINE)
+        assert st1[0]["tile_shape"] == want_one, (n, st1)
 
 
 @pytest.mark.parametrize("precision", [PREC_FP64, PREC_FP32_REFINE])
