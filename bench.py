@@ -435,11 +435,12 @@ def main():
     solve_gbs = solve_bytes / (ms_step * 1e-3) / 1e9
     # The smoother's per-launch working set: every field's pass bytes at once
     # (the three launches run concurrently). Within the 256 MB Infinity Cache
-    # the sweeps are VALU-bound (SQ_INSTS_VALU at 0.84 of the roof at 1024^2);
-    # beyond it each launch streams from HBM as well.
+    # the sweeps are bound by VALU work and the tile loads' latency
+    # (SQ_INSTS_VALU at 0.73 of the roof at 1024^2); beyond it each launch
+    # streams from HBM as well.
     working_set = sum(s_vox * bytes_per_voxel_pass(fc, "fp32") for s_vox, fc in zip(span, fcs))
-    binding = ("VALU (FP32 register-blocked smoother; working set %.0f MB, inside the 256 MB "
-               "Infinity Cache)" % (working_set / 1e6) if working_set <= 256e6 else
+    binding = ("VALU and tile-load latency (FP32 register-blocked smoother; working set %.0f MB, "
+               "inside the 256 MB Infinity Cache)" % (working_set / 1e6) if working_set <= 256e6 else
                "HBM and VALU (FP32 register-blocked smoother; working set %.0f MB, beyond the "
                "256 MB Infinity Cache)" % (working_set / 1e6))
 
