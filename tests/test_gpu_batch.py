@@ -238,7 +238,7 @@ def test_a_concurrent_batch_reports_a_field_that_does_not_settle():
 def test_a_batch_whose_fields_stop_at_different_times_keeps_their_own_bits(graph):
     """Eight fields of different sizes to a relative residual: as they stop
     the batch is laid out again over the rest, and every field still gets
-    the sweeps and bits it gets alone."""
+    the sweeps and bits it gets alone at the tile shape the batch chose."""
     say(f"case uneven batch graph={graph}")
     hist = cases.campaign_histories(seed=4400, count=4, lo=48, hi=200)
     fcs = [fc for h in hist for fc in h]
