@@ -1209,29 +1209,92 @@ __global__ __launch_bounds__(NW * 64) void relax3d_stream(PassArgs a_in) {
     }
 }
 
-// Two red-black sweeps per pass, for the FP32 volume smoother: sweep A runs
-// one plane ahead of sweep B in the same z stream, so each pass over memory
-// advances every voxel twice. At loop step z: A red of plane z, A black of
-// z-1, B red of z-2, B black of z-3, then plane z-3 is stored. Each phase
-// reads only what the sweep order allows: A red(z) the original black cells
-// of z+1 and the not-yet-A-black ones of z-1; A black(z-1) the A-red cells of
-// z-2 (B has not touched them yet) and z; B red(z-2) the A-black cells of
-// z-3 and z-1; B black(z-3) the B-red cells of z-4 and z-2. The halo is four
-// cells wide in x and y (two per sweep) and the A sweep covers the chunk plus
-// three planes either side, so the stored 120 x (NW*NR - 8) x zc block is
-// exact. FP32 smoothing only: no change tracking, no stop inside a pass.
-template <int NW, int NR>
-__global__ __launch_bounds__(NW * 64) void relax3d_stream2(PassArgs a_in) {
+// S red-black sweeps per pass, for the FP32 volume smoother, as a pipeline
+// of 2S half-sweep phases in one z stream: at loop step z, phase h (colour
+// h & 1 of sweep h / 2) updates plane z - h, in the order h = 0, 1, ..., so
+// every phase reads its plane's z neighbours at the level the sequential
+// sweeps give them (plane z-h+1 was just updated by phase h-1, plane z-h-1
+// not yet by phase h+1). Plane z-2S+1 then holds its final values and is
+// stored. The halo is 2S cells in x and y; phase h covers the chunk plus
+// 2S-1-h planes either side, so the stored (128-4S) x (NW*NR-4S) x zc block
+// is exact, and red-black SOR fixes every cell's operands: the bits are the
+// one-sweep smoother's.
+//
+// One barrier a step. A wave's rows at the edge of its block need the
+// neighbouring waves' edge rows, and every phase takes them from the step
+// before: phase h of step z needs the other colour of plane z-h as phase h-1
+// left it, and phase h-1 updated plane z-h at step z-1 (phase 0 needs plane
+// z's original values, published at step z-1 when that plane is the next
+// one). So each wave publishes its edge rows after every phase into the slot
+// of its step, reads those of the previous step, and one barrier at the end
+// of the step orders the two (double-buffered by step parity). The next
+// plane's loads are issued a step ahead.
+// A workgroup barrier that orders LDS only: __syncthreads() also waits for
+// every global store of the wave to complete (its release fence), and on this
+// target a wait on stores is a wait on the vector-memory counter, which the
+// loads in flight share. Kernels that prefetch across a barrier use this one.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Slot of plane z+2-j in a ring of L planes at a step with z mod L == R.
+template <int L, int R, int J>
+__device__ __forceinline__ constexpr int ring_slot() { return ((R + 2 - J) % L + 2 * L) % L; }
+
+template <int I, int N, typename F>
+__device__ __attribute__((always_inline)) void static_for(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// The FP32 volume smoother's update of one colour of a plane whose cells are
+// kept by column parity (st[q][i]: row i, column 2l + q), the colour's parity
+// on row 0 being P0, with alpha = omega d / (real d + decay) given per parity
+// and row: the same arithmetic, in the same order, as plane_update<float,
+// false, false>, so the same bits. The other colour's cells in the row are
+// the lane's other parity; the cells above, below and in the planes either
+// side are at the same parity.
+template <int NR, int P0>
+__device__ __forceinline__ void pipe_update(Plane<float, NR>& cur, const Plane<float, NR>& below_z,
+                                            const Plane<float, NR>& above_z,
+                                            const PlaneK<float, NR>& K, float above, float below,
+                                            const float (&al)[2][NR], float omo) {
+#pragma unroll
+    for (int i = 0; i < NR; i++) {
+        const int q = P0 ^ (i & 1);
+        const float o = cur.st[q ^ 1][i];
+        const float across = q ? from_right(o) : from_left(o);
+        const float up = i > 0 ? cur.st[q][i - 1] : above;
+        const float dn = i < NR - 1 ? cur.st[q][i + 1] : below;
+        const float zm = below_z.st[q][i];
+        const float zp = above_z.st[q][i];
+        const float kq = K.k[q][i];
+        const float before = cur.st[q][i];
+        const float sum = ((across + o) + (up + dn)) + (zm + zp);
+        float after = fma_t(omo, before, fma_t(al[q][i], sum, kq));
+        after = max_t(after, K.u[q][i]);
+        cur.st[q][i] = kq == kq ? after : before;
+    }
+}
+
+template <int S, int NW, int NR, bool LB>
+__global__ __launch_bounds__(NW * 64) void relax3d_pipe(PassArgs a_in) {
     using T = float;
-    using T2 = float2;
     const PassArgs a = resolve(a_in);
     constexpr int RY = NW * NR;
-    constexpr int TX = 120, TY = RY - 8;
-    static_assert(NR >= 2 && TY > 0, "tile shape");
-    __shared__ T edge[2][NW][2][64];  // [slot][wave][first/last row][lane]: one colour
+    constexpr int HALO = 2 * S;
+    constexpr int TX = 128 - 2 * HALO, TY = RY - 2 * HALO;
+    constexpr int NPH = 2 * S;        // phases a step
+    constexpr int L = NPH + 4;        // ring length (even): planes z+2 .. z-2S live at step z
+    static_assert(NR >= 2 && NR % 2 == 0 && TY > 0 && S >= 1, "tile shape");
+    __shared__ T edge[2][NPH][NW][2][64];  // [step parity][phase][wave][first/last row][lane]
 
     const int lane = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int p = a.tile_prob[t];
     const Prob& pr = a.probs[p];
@@ -1240,129 +1303,215 @@ __global__ __launch_bounds__(NW * 64) void relax3d_stream2(PassArgs a_in) {
     const int nxy = pr.ntx[tl] * pr.nty[tl];
     const int ch = lt / nxy, txy = lt % nxy;
     const int ti = txy % pr.ntx[tl], tj = txy / pr.ntx[tl];
-    const int gxo = pr.xs + ti * TX - 4;
-    const int gyo = pr.ys + tj * TY - 4;
+    // Regions start on even columns and rows (pr.xs is even; the y tiling
+    // of this kernel starts at the even row at or before the span's first),
+    // so column 2l of row 0 of plane z has colour z & 1 in every region.
+    const int gxo = pr.xs + ti * TX - HALO;
+    const int gyo = (pr.ys & ~1) + tj * TY - HALO;
     const int za = pr.zs + ch * pr.zc;
     const int zb = za + pr.zc - 1 < pr.z1 ? za + pr.zc - 1 : pr.z1;
     const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
-    const size_t plane = (size_t)P * Hh;
+    // byte offsets fit 32 bits (the host takes this kernel only for arrays
+    // under 4 GiB), so every load is a base plus a 32-bit lane offset
+    const uint32_t plane_bytes = (uint32_t)P * (uint32_t)Hh * 4u;
     const int gx = gxo + 2 * lane;
     const bool colin = gx >= 0 && gx < W;
-    const T* kk = (const T*)pr.k;
-    const T* lbp = (const T*)pr.lb;
-    const bool has_lb = lbp != nullptr;
+    typedef float F2 __attribute__((ext_vector_type(2)));
+    using GB = __attribute__((address_space(1))) const unsigned char;
+    using GT2 = __attribute__((address_space(1))) const F2;
+    typedef __attribute__((address_space(1))) F2 GF2;
+    // Pointers in the global address space: their loads count against the
+    // vector-memory counter only, so LDS waits do not wait for the prefetch.
+    GB* kk = (GB*)pr.k;
+    GB* lbp = (GB*)pr.lb;  // LB: the launch's fields have lower bounds
+    const bool has_lb = LB && lbp != nullptr;
+    // the problem's bounds in registers: the stores below would otherwise
+    // make the compiler reload them (and wait for every load in flight)
+    const int zlo = pr.zs, zhi = pr.z1, xhi = pr.x1, yhi = pr.y1;
     const T d = (T)pr.d, decay = (T)pr.decay, omega = (T)pr.omega, omo = (T)pr.omo;
     const int r0 = wv * NR;
     const int rx[2] = {(gx > 0) + (gx < W - 1), (gx + 1 > 0) + (gx + 1 < W - 1)};
     int ry[NR];
     bool rowin[NR];
+    uint32_t rowoff[NR];  // byte offset of the lane's pair in its row of a plane
 #pragma unroll
     for (int i = 0; i < NR; i++) {
         const int gy = gyo + r0 + i;
         ry[i] = (gy > 0) + (gy < Hh - 1);
         rowin[i] = colin && gy >= 0 && gy < Hh;
+        rowoff[i] = rowin[i] ? ((uint32_t)gy * (uint32_t)P + (uint32_t)gx) * 4u : 0u;
     }
-    const int base0 = (gxo + gyo + r0) & 1;  // colour of column 2l, row 0, plane 0
-    const T* src = (const T*)pr.x[a.pass & 1];
+    // alpha of interior planes (z neighbours on both sides) per parity and row
+    T a2[2][NR];
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int i = 0; i < NR; i++) a2[q][i] = omega * d * rcp_t((T)(rx[q] + ry[i] + 2) * d + decay);
+    GB* src = (GB*)pr.x[a.pass & 1];
 
-    auto load_state = [&](Plane<T, NR>& pl, int z) {
-        const int sw0 = (base0 + z) & 1;
+    // Raw loads of a plane's pairs as stored (a row outside the field reads
+    // element 0); the masks are applied when the plane is sorted, a step
+    // later, so nothing waits for a load before its plane is needed.
+    auto load_state = [&](F2 (&v)[NR], int z) __attribute__((always_inline)) {
+        const bool zin = z >= 0 && z < D;
+        const uint32_t zo = zin ? (uint32_t)z * plane_bytes : 0u;
+#pragma unroll
+        for (int i = 0; i < NR; i++) v[i] = *(GT2*)(src + (zin ? zo + rowoff[i] : 0u));
+    };
+    auto load_coef = [&](F2 (&c)[NR], F2 (&u)[NR], int z) __attribute__((always_inline)) {
+        const bool zin = z >= 0 && z < D;
+        const uint32_t zo = zin ? (uint32_t)z * plane_bytes : 0u;
 #pragma unroll
         for (int i = 0; i < NR; i++) {
-            T ve = T(0), vo = T(0);
-            if (rowin[i] && z >= 0 && z < D) {
-                const T2 v = *(const T2*)(src + (size_t)z * plane + (size_t)(gyo + r0 + i) * P + gx);
-                ve = v.x; vo = v.y;
-            }
-            const bool sw = (sw0 ^ (i & 1)) != 0;
-            pl.st[0][i] = sw ? vo : ve;
-            pl.st[1][i] = sw ? ve : vo;
+            const uint32_t off = zin ? zo + rowoff[i] : 0u;
+            c[i] = *(GT2*)(kk + off);
+            if (has_lb) u[i] = *(GT2*)(lbp + off);
         }
     };
-    auto load_coef = [&](PlaneK<T, NR>& K, int z) {
-        const int sw0 = (base0 + z) & 1;
+    // Cells stay by column parity as loaded (st[q][i]: row i, column 2l + q);
+    // sorting a plane is only masking what lies outside the field.
+    auto sort_state = [&](Plane<T, NR>& pl, const F2 (&v)[NR], int z) __attribute__((always_inline)) {
+        const bool zin = z >= 0 && z < D;
 #pragma unroll
         for (int i = 0; i < NR; i++) {
-            T ce = nan_of<T>(), co = nan_of<T>(), ue = neg_inf<T>(), uo = ue;
-            if (rowin[i] && z >= 0 && z < D) {
-                const size_t off = (size_t)z * plane + (size_t)(gyo + r0 + i) * P + gx;
-                const T2 c = *(const T2*)(kk + off);
-                ce = c.x; co = c.y;
-                if (has_lb) { const T2 u = *(const T2*)(lbp + off); ue = u.x; uo = u.y; }
-            }
-            const bool sw = (sw0 ^ (i & 1)) != 0;
-            K.k[0][i] = sw ? co : ce; K.k[1][i] = sw ? ce : co;
-            K.u[0][i] = sw ? uo : ue; K.u[1][i] = sw ? ue : uo;
+            const bool in = rowin[i] && zin;
+            pl.st[0][i] = in ? v[i].x : T(0);
+            pl.st[1][i] = in ? v[i].y : T(0);
+        }
+    };
+    auto sort_coef = [&](PlaneK<T, NR>& K, const F2 (&c)[NR], const F2 (&u)[NR], int z) __attribute__((always_inline)) {
+        const bool zin = z >= 0 && z < D;
+#pragma unroll
+        for (int i = 0; i < NR; i++) {
+            const bool in = rowin[i] && zin;
+            K.k[0][i] = in ? c[i].x : nan_of<T>();
+            K.k[1][i] = in ? c[i].y : nan_of<T>();
+            K.u[0][i] = has_lb && in ? u[i].x : neg_inf<T>();
+            K.u[1][i] = has_lb && in ? u[i].y : neg_inf<T>();
         }
     };
 
-    // planes z-4 .. z+1 at loop step z, and the coefficients of z-3 .. z;
-    // the first steps run A red only, so only planes zs0-1, zs0 and the
-    // coefficients of zs0 start loaded
-    Plane<T, NR> q4{}, q3{}, q2{}, q1{}, q0{}, n1{};
-    PlaneK<T, NR> k3{}, k2{}, k1{}, k0{}, kn{};
-    const int zs0 = za - 3;
-    load_state(q1, zs0 - 1);
-    load_state(q0, zs0);
-    load_coef(k0, zs0);
+    // A ring of L planes: plane p sits in slot (p - zr) mod L, so with the
+    // step's (z - zr) mod L a compile-time constant (the loop below is
+    // unrolled L times) every slot index is one and no register moves. zr is
+    // the even plane at or before the first step z0; the loop starts there
+    // and runs whole blocks of L steps, the steps outside [z0, zend] loading,
+    // updating and storing nothing. Every step sorts its planes whatever it
+    // does, so no slot keeps a stale plane alive across the loop. At step z
+    // the ring holds the states of planes z+2 (sorted at the step's start,
+    // loaded a step before) .. z-2S and the coefficients of planes z ..
+    // z-2S+1 (plane z's sorted at the step's start, loaded a step before).
+    // Phase h starts at step z0 + h.
+    const int z0 = za - (NPH - 1);
+    const int zend = zb + (NPH - 1);
+    const int zr = z0 & ~1;
+    Plane<T, NR> q[L];
+    PlaneK<T, NR> K[L];
+    F2 nv[NR], nc[NR], nu[NR] = {};
+    {
+        F2 v[NR];
+        load_state(v, zr + 1); sort_state(q[ring_slot<L, 0, 1>()], v, zr + 1);
+        load_state(v, zr); sort_state(q[ring_slot<L, 0, 2>()], v, zr);
+        load_state(v, zr - 1); sort_state(q[ring_slot<L, 0, 3>()], v, zr - 1);
+        // plane zr's original colour 1 (parity 1 on row 0: zr is even) for
+        // phase 0 of step zr
+        const Plane<T, NR>& first = q[ring_slot<L, 0, 2>()];
+        edge[1][0][wv][0][lane] = first.st[1][0];
+        edge[1][0][wv][1][lane] = first.st[0][NR - 1];
+    }
+    load_state(nv, zr + 2);
+    load_coef(nc, nu, zr);
 
     int par;
     if (!sweep_prologue(a, pr, p, lt, &par)) return;
-    T* dst = (T*)pr.x[par ^ 1];
+    __attribute__((address_space(1))) unsigned char* dst =
+        (__attribute__((address_space(1))) unsigned char*)pr.x[par ^ 1];
+    lds_barrier();
 
-    unsigned long long mv = 0ull;
-    int slot = 0;
-    // colour C of plane z: publish the other colour's edge rows, then update
-    auto phase = [&](int C, Plane<T, NR>& cur, const Plane<T, NR>& bz, const Plane<T, NR>& az,
-                     const PlaneK<T, NR>& K, int z) {
-        edge[slot][wv][0][lane] = cur.st[C ^ 1][0];
-        edge[slot][wv][1][lane] = cur.st[C ^ 1][NR - 1];
-        __syncthreads();
-        const T above = wv > 0 ? edge[slot][wv - 1][1][lane] : T(0);
-        const T below = wv < NW - 1 ? edge[slot][wv + 1][0][lane] : T(0);
-        slot ^= 1;
+    // Phase h of a step: colour h & 1 of plane z, whose cells of that colour
+    // sit at parity P0 on row 0.
+    auto phase = [&](auto p0c, int h, Plane<T, NR>& cur, const Plane<T, NR>& bz,
+                     const Plane<T, NR>& az, const PlaneK<T, NR>& Kc, int z, int rd, int wr)
+        __attribute__((always_inline)) {
+        constexpr int P0 = decltype(p0c)::value;
+        const T above = edge[rd][h][wv > 0 ? wv - 1 : wv][1][lane];
+        const T below = edge[rd][h][wv < NW - 1 ? wv + 1 : wv][0][lane];
+        const T ab = wv > 0 ? above : T(0);
+        const T bl = wv < NW - 1 ? below : T(0);
         const int rz = (z > 0) + (z < D - 1);
-        const int pc = ((base0 + z) & 1) ^ C;  // x parity of colour C on row 0
-        if (C == 0) {
-            if (pc) plane_update<T, false, false, NR, 0, 1>(cur, bz, az, K, above, below, lane, rx, ry, rz, d, decay, omega, omo, T(0), false, r0, 4, 4 + TY, &mv);
-            else plane_update<T, false, false, NR, 0, 0>(cur, bz, az, K, above, below, lane, rx, ry, rz, d, decay, omega, omo, T(0), false, r0, 4, 4 + TY, &mv);
+        if (rz == 2) {
+            pipe_update<NR, P0>(cur, bz, az, Kc, ab, bl, a2, omo);
         } else {
-            if (pc) plane_update<T, false, false, NR, 1, 1>(cur, bz, az, K, above, below, lane, rx, ry, rz, d, decay, omega, omo, T(0), false, r0, 4, 4 + TY, &mv);
-            else plane_update<T, false, false, NR, 1, 0>(cur, bz, az, K, above, below, lane, rx, ry, rz, d, decay, omega, omo, T(0), false, r0, 4, 4 + TY, &mv);
+            // a plane on a z face of the field: fewer real neighbours
+            T af[2][NR];
+#pragma unroll
+            for (int qq = 0; qq < 2; qq++)
+#pragma unroll
+                for (int i = 0; i < NR; i++)
+                    af[qq][i] = omega * d * rcp_t((T)(rx[qq] + ry[i] + rz) * d + decay);
+            pipe_update<NR, P0>(cur, bz, az, Kc, ab, bl, af, omo);
+        }
+        // this colour of the plane as the next phase reads it at the next step
+        if (h + 1 < NPH) {
+            edge[wr][h + 1][wv][0][lane] = cur.st[P0][0];
+            edge[wr][h + 1][wv][1][lane] = cur.st[P0 ^ 1][NR - 1];
         }
     };
-    auto in_box = [&](int z) { return z >= pr.zs && z <= pr.z1; };
 
-    for (int z = zs0; z <= zb + 3; z++) {
-        load_state(n1, z + 1);
-        load_coef(kn, z + 1);
-        if (in_box(z)) phase(0, q0, q1, n1, k0, z);                                 // A red z
-        if (z - 1 >= za - 2 && in_box(z - 1)) phase(1, q1, q2, q0, k1, z - 1);      // A black z-1
-        if (z - 2 >= za - 1 && z - 2 <= zb + 1 && in_box(z - 2))
-            phase(0, q2, q3, q1, k2, z - 2);                                        // B red z-2
-        if (z - 3 >= za && z - 3 <= zb) {
-            phase(1, q3, q4, q2, k3, z - 3);                                        // B black z-3
-            // store plane z-3: interior rows and pairs inside the swept box
-            if (lane >= 2 && lane < 62 && gx <= pr.x1) {
-                const int zz = z - 3;
-                const int sw0 = (base0 + zz) & 1;
+    // Step z, with (z - zr) mod L == RC: every ring slot is a compile-time
+    // constant, and so is every colour's parity, since zr and L are even:
+    // plane z-h's colour h & 1 sits at parity (z - h + h) & 1 == RC & 1 on
+    // row 0 for every h.
+    auto step = [&](auto rc, int z) __attribute__((always_inline)) {
+        constexpr int RC = decltype(rc)::value;
+        constexpr int P0 = RC & 1;
+        const int rd = (z - 1) & 1, wr = z & 1;
+        const bool live = z >= z0 && z <= zend;
+        // the state loaded a step ago becomes plane z+2, the coefficients
+        // plane z's; the next ones go in flight while some step still needs
+        // them (plane p's state from step p-1, its coefficients from step p)
+        sort_state(q[ring_slot<L, RC, 0>()], nv, z + 2);
+        sort_coef(K[ring_slot<L, RC, 2>()], nc, nu, z);
+        if (z + 3 <= zend + 1) load_state(nv, z + 3);
+        if (z + 1 <= zend) load_coef(nc, nu, z + 1);
+        // plane z+1's original colour 1 (parity P0 on row 0), for phase 0 of
+        // the next step
+        {
+            const Plane<T, NR>& nx = q[ring_slot<L, RC, 1>()];
+            edge[wr][0][wv][0][lane] = nx.st[P0][0];
+            edge[wr][0][wv][1][lane] = nx.st[P0 ^ 1][NR - 1];
+        }
+        static_for<0, NPH>([&](auto hc) __attribute__((always_inline)) {
+            constexpr int h = decltype(hc)::value;
+            const int zh = z - h;
+            if (live && zh >= zlo && zh <= zhi && zh >= za - (NPH - 1 - h))
+                phase(std::integral_constant<int, P0>{}, h, q[ring_slot<L, RC, h + 2>()],
+                      q[ring_slot<L, RC, h + 3>()], q[ring_slot<L, RC, h + 1>()],
+                      K[ring_slot<L, RC, h + 2>()], zh, rd, wr);
+        });
+        // plane z-2S+1 is final: interior rows and pairs inside the swept box
+        const int zs = z - (NPH - 1);
+        if (zs >= za && zs <= zb && lane >= S && lane < 64 - S && gx <= xhi) {
+            const Plane<T, NR>& fin = q[ring_slot<L, RC, NPH + 1>()];
 #pragma unroll
-                for (int i = 0; i < NR; i++) {
-                    const int r = r0 + i;
-                    if (r < 4 || r >= 4 + TY) continue;
-                    const int gy = gyo + r;
-                    if (gy > pr.y1) continue;
-                    const bool sw = (sw0 ^ (i & 1)) != 0;
-                    T2 v;
-                    v.x = sw ? q3.st[1][i] : q3.st[0][i];
-                    v.y = sw ? q3.st[0][i] : q3.st[1][i];
-                    *(T2*)(dst + (size_t)zz * plane + (size_t)gy * P + gx) = v;
-                }
+            for (int i = 0; i < NR; i++) {
+                const int r = r0 + i;
+                if (r < HALO || r >= HALO + TY) continue;
+                const int gy = gyo + r;
+                if (gy > yhi) continue;
+                F2 v;
+                v.x = fin.st[0][i];
+                v.y = fin.st[1][i];
+                *(GF2*)(dst + ((uint32_t)zs * plane_bytes + rowoff[i])) = v;
             }
         }
-        q4 = q3; q3 = q2; q2 = q1; q1 = q0; q0 = n1;
-        k3 = k2; k2 = k1; k1 = k0; k0 = kn;
-    }
+        lds_barrier();
+    };
+
+    for (int zz = zr; zz <= zend; zz += L)
+        static_for<0, L>([&](auto rc) __attribute__((always_inline)) {
+            step(rc, zz + decltype(rc)::value);
+        });
 }
 
 // ---------------------------------------------------------------------------

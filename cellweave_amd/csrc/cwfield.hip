@@ -91,17 +91,19 @@ constexpr double kPi = 3.14159265358979323846;
 // 3D shapes: kind 0 = in-place red and black half sweeps (half3d, two
 // launches per sweep); kind 1 = z-streaming relax3d_stream with NW waves of
 // NR rows and zc planes per chunk (one launch per sweep, ping-pong); kind 2 =
-// the same with two sweeps per launch (below).
+// the pipelined FP32 smoother relax3d_pipe, two sweeps per launch (below).
 struct Shape3 { int kind, nw, nr, zc; };
 constexpr Shape3 k3DShapes[] = {{0, 4, 1, 1},  {1, 8, 4, 16}, {1, 4, 8, 16}, {1, 8, 4, 32},
                                  {1, 8, 4, 8},  {1, 8, 4, 24}, {1, 4, 8, 32}, {1, 8, 4, 48},
                                  {1, 4, 8, 24}, {1, 8, 4, 4},  {1, 4, 8, 8},  {1, 4, 8, 4},
                                  {2, 16, 2, 16}, {2, 16, 2, 32}, {2, 16, 2, 8}, {1, 8, 4, 2},
-                                 {1, 4, 8, 2},  {1, 8, 4, 1}};
-// kind 2: two sweeps per launch (relax3d_stream2, FP32 smoothing only), 16
+                                 {1, 4, 8, 2},  {1, 8, 4, 1},  {2, 16, 2, 0}};
+// kind 2: two sweeps per launch (relax3d_pipe<2>, FP32 smoothing only), 16
 // waves of 2 rows; its remainder launches use the 8 x 4 one-sweep kernel,
-// which has the same 32-row region
+// which has the same 32-row region. zc 0: the chunk length that fits the
+// plan's workgroups on the CUs in the fewest rounds (pipe_chunks).
 constexpr bool two_sweep(int shape3) { return k3DShapes[shape3].kind == 2; }
+constexpr int kVolumePipeAuto = 18;
 constexpr int kNum3DShapes = (int)(sizeof(k3DShapes) / sizeof(k3DShapes[0]));
 constexpr int kDefault3DShape = 1;
 
@@ -160,8 +162,24 @@ long volume_tiles(cw_field* const* F, uint32_t nf, int si, uint64_t* voxels) {
     }
     return tiles;
 }
+// relax3d_pipe addresses a field's FP32 arrays with 32-bit byte offsets
+bool pipe_addressable(const cw_field* f) {
+    return (uint64_t)f->P * f->H * f->D * sizeof(float) < (1ull << 32);
+}
+
+// FP32 volumes of 2^18 voxels or more take the pipelined two-sweep smoother
+// with its automatic chunks (kVolumePipeAuto): FP32+refine to 1e-8 on the
+// spheroids, ms (one-sweep rule above / two-sweep, automatic chunks):
+//   96^3 3.46 / 2.41   128^3 8.53 / 4.63   160^3 11.9 / 7.21
+//   192^3 21.2 / 10.4   256^3 50.8 / 30.1 (the same bits, sweep for sweep)
 int default_volume_shape(cw_field* const* F, uint32_t nf, bool fp64) {
     uint64_t voxels = 0;
+    if (!fp64) {
+        bool fits = true;
+        for (uint32_t i = 0; i < nf; i++) fits = fits && pipe_addressable(F[i]);
+        volume_tiles(F, nf, kVolumeChunk16, &voxels);
+        if (fits && voxels >= (1u << 18)) return kVolumePipeAuto;
+    }
     if (fp64) {
         for (int c : {kVolumeChunk16, kVolumeChunk8})
             if (volume_tiles(F, nf, c, &voxels) >= 220) return c;
@@ -313,6 +331,8 @@ struct Plan {
     bool three_d = false;
     bool lower_bound = true;  // FP32 plane smoothing: some problem has a lower bound
     int shape3 = kDefault3DShape;   // k3DShapes index for volumes
+    int pipe_k = 1;                 // z chunks of each volume in the automatic two-sweep shape
+    bool pipe_fits = true;          // every FP32 array of the plan under 4 GiB (relax3d_pipe)
     bool in_place() const { return three_d && k3DShapes[shape3].kind == 0; }
     std::vector<Prob> hp;
     std::vector<Ctl> hc;
@@ -389,6 +409,7 @@ struct Plan {
         use_graph = false;
         run_host[0] = run_host[1] = 0;
         compact = false; act.clear(); nact = 0;
+        pipe_k = 1; pipe_fits = true;
     }
 
     ~Plan() {
@@ -426,6 +447,41 @@ Plan& field_plan(cw_field* f, int which) {
     return p;
 }
 
+// z chunks per volume for the automatic two-sweep shape. A relax3d_pipe
+// workgroup (16 waves, 112 VGPRs at most: one a CU) streams its chunk of zc
+// planes in zc + 7 steps of about the same time, so a launch takes
+// rounds x (zc + 7) steps, rounds = ceil(workgroups / CUs): the chunk count k
+// that minimises ceil(columns k / CUs) (ceil(planes / k) + 7) over the plan's
+// active volumes (256^3: 33 columns, k = 7, 37-plane chunks, one round).
+int pipe_chunks(const Plan& pl, const std::vector<char>* active) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        cus = 256;
+    long cols = 0;
+    int depth = 1;
+    for (int p = 0; p < pl.nprob; p++) {
+        if (active && !(*active)[p]) continue;
+        const Prob& pr = pl.hp[p];
+        const int sx = pr.x1 - pr.xs + 1, sy = pr.y1 - (pr.ys & ~1) + 1;
+        cols += (long)((sx + 119) / 120) * ((sy + 23) / 24);
+        depth = std::max(depth, pr.z1 - pr.zs + 1);
+    }
+    if (cols <= 0) return 1;
+    int best = 1;
+    long best_cost = -1;
+    for (int k = 1; k <= depth; k++) {
+        const long rounds = (cols * k + cus - 1) / cus;
+        const long cost = rounds * ((depth + k - 1) / k + 7);
+        if (best_cost < 0 || cost < best_cost) {
+            best = k;
+            best_cost = cost;
+        }
+    }
+    return best;
+}
+
 // Lays out the tilings of every problem (s-sweep tile, one-sweep tile,
 // elementwise tile) and uploads problems, control blocks and tables.
 // With `active`, problems marked 0 get no tiles: launches skip them without
@@ -434,6 +490,8 @@ Plan& field_plan(cw_field* f, int which) {
 cw_status plan_upload(Plan& pl, const std::vector<char>* active = nullptr) {
     const int np = pl.nprob;
     for (int tl = 0; tl < 3; tl++) pl.tp[tl].clear();
+    if (pl.three_d && two_sweep(pl.shape3) && k3DShapes[pl.shape3].zc == 0)
+        pl.pipe_k = pipe_chunks(pl, active);
     for (int p = 0; p < np; p++) {
         Prob& pr = pl.hp[p];
         const int sx = pr.x1 - pr.xs + 1, sy = pr.y1 - pr.ys + 1, sz = pr.z1 - pr.zs + 1;
@@ -449,12 +507,16 @@ cw_status plan_upload(Plan& pl, const std::vector<char>* active = nullptr) {
                 if (sh.kind == 0) {
                     nx = ny = nz = 0;
                 } else {
-                    // two sweeps need a halo of 4 (one sweep 2) in x and y
+                    // two sweeps need a halo of 4 (one sweep 2) in x and y;
+                    // the two-sweep kernel's y tiling starts on an even row
                     const bool two = sh.kind == 2 && tl == TL_MAIN;
                     const int tx = two ? 120 : 124;
                     const int ty = sh.nw * sh.nr - (two ? 8 : 4);
-                    nx = (sx + tx - 1) / tx; ny = (sy + ty - 1) / ty; nz = (sz + sh.zc - 1) / sh.zc;
-                    pr.zc = sh.zc;
+                    const int syk = two ? pr.y1 - (pr.ys & ~1) + 1 : sy;
+                    // one chunk length for both tilings (the kernels share pr.zc)
+                    const int zc = sh.zc == 0 ? (sz + pl.pipe_k - 1) / pl.pipe_k : sh.zc;
+                    nx = (sx + tx - 1) / tx; ny = (syk + ty - 1) / ty; nz = (sz + zc - 1) / zc;
+                    pr.zc = zc;
                 }
             } else {
                 const int si = tl == TL_MAIN ? pl.shape : 0;
@@ -640,7 +702,10 @@ cw_status launch_stream3d(Plan& pl, const PassArgs& a) {
 #define CW_L(kern, blk) hipLaunchKernelGGL(kern, grid, blk, 0, st, a)
     if (two_sweep(pl.shape3) && a.tl == TL_MAIN) {
         if (pl.fp64) return fail(CW_ERR_ARG, "two-sweep volume shapes are for FP32 smoothing only");
-        CW_L((relax3d_stream2<16, 2>), b1024);
+        if (!pl.pipe_fits)
+            return fail(CW_ERR_ARG, "two-sweep volume shapes address arrays under 4 GiB only");
+        if (pl.lower_bound) CW_L((relax3d_pipe<2, 16, 2, true>), b1024);
+        else CW_L((relax3d_pipe<2, 16, 2, false>), b1024);
     } else if (pl.fp64 && pl.plain) {
         if (pl.general) {
             if (w4) CW_L((relax3d_stream<double, 4, 8, true, true>), b256);
@@ -1273,6 +1338,8 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
         inner.shape3 = (o.shape >= 0 && o.shape < kNum3DShapes) ? o.shape
                                                                  : default_volume_shape(F, nf, false);
         inner.s_main = two_sweep(inner.shape3) ? 2 : 1;
+        for (uint32_t i = 0; i < nf; i++)
+            if (!pipe_addressable(F[i])) inner.pipe_fits = false;
     }
     const int s_used = inner.s_main;
     inner.lower_bound = false;

@@ -77,11 +77,13 @@ def test_fp64_planes_sweep_deep_launches():
 
 
 @pytest.mark.parametrize("n,fp64_shape,fp32_shape",
-                         [(64, 17, 17), (80, 17, 17), (96, 15, 15), (120, 9, 15), (160, 4, 9)])
+                         [(64, 17, 17), (80, 17, 18), (96, 15, 18), (120, 9, 18), (160, 4, 18)])
 def test_volume_chunk_follows_workgroups(n, fp64_shape, fp32_shape):
     """Volumes take the longest z chunk that still gives the batch enough
     workgroups: FP64 16 or 8 planes at 220, else 4, 2 or 1 plane at 150;
-    FP32 16 to 2 planes at 256, else 2 or 1 plane at 150."""
+    FP32 spans of 2^18 voxels or more the two-sweep pipelined smoother with
+    automatic chunks (shape 18), smaller ones 16 to 2 planes at 256, else 2
+    or 1 plane at 150."""
     say(f"case volume chunk {n}^3")
     (fc,) = cases.spheroid(n=n, ncells=int(round(60000 * (n / 256) ** 3)), radius=110 * n / 256,
                            necrotic=40 * n / 256)

@@ -247,7 +247,7 @@ def test_fp32_refine_reaches_residual_oxygen_200(shape):
 
 
 SHAPES_3D = [*range(12), 15, 16, 17]  # in-place half sweeps (0) and z-streaming variants
-SMOOTH_SHAPES_3D = range(18)  # and the two-sweep FP32 smoothers (12-14)
+SMOOTH_SHAPES_3D = range(19)  # and the two-sweep FP32 smoothers (12-14, 18)
 
 
 def ball_case(side=31, r=10.0, u=0.006, general=False):
@@ -384,7 +384,7 @@ def test_two_sweep_volume_shapes_are_refused_for_fp64():
         g.relax_ex(omega=1.8, criterion=CRIT_MAX_CHANGE, tolerance=1e-10, tile_shape=12)
 
 
-@pytest.mark.parametrize("two", [12, 13, 14])
+@pytest.mark.parametrize("two", [12, 13, 14, 18])
 def test_two_sweep_smoother_gives_the_one_sweep_bits(two):
     """Red-black SOR fixes every cell's operands, whatever order the stream
     visits planes in: the two-sweep FP32 volume smoother must give the bits
@@ -400,3 +400,68 @@ def test_two_sweep_smoother_gives_the_one_sweep_bits(two):
     assert s2["sweeps_per_pass"] == 2
     assert s1["sweeps"] == s2["sweeps"]
     assert bits_equal(g1.download(), g2.download())
+
+
+@pytest.mark.parametrize("two", [12, 18])
+@pytest.mark.parametrize("dims", [(40, 36, 30), (131, 57, 23), (70, 90, 41)])
+def test_two_sweep_smoother_on_ragged_boxes(two, dims):
+    """Boxes held on their faces, so the free span starts on odd columns,
+    rows and planes (the two-sweep kernel's regions start on even ones) and
+    ends ragged against every region and chunk: the bits of the one-sweep
+    smoother, sweep for sweep, with and without a lower bound."""
+    w, h, d = dims
+    say(f"case two-sweep shape {two} on a {dims} box")
+    rng = np.random.default_rng(w + h + d)
+    faces = np.zeros((d, h, w), np.uint8)
+    faces[:, 0, :] = faces[:, -1, :] = faces[:, :, 0] = faces[:, :, -1] = 1
+    faces[0] = faces[-1] = 1
+    for sinks in (True, False):
+        up = np.where(faces.reshape(-1) == 1, 0.0, rng.uniform(0, 2e-3, w * h * d)) if sinks \
+            else np.zeros(w * h * d)
+        gs = []
+        for _ in range(2):
+            ref = OracleField(w, h, d, 0.25, 0.0 if sinks else 1e-3, 1.0)
+            ref.uptake[:] = up
+            ref.hold_where(1.0, faces.reshape(-1))
+            ref.copy_next_from_values()
+            gs.append(gpu_from_oracle(ref))
+        kw = dict(omega=1.9, criterion=CRIT_REL_RESIDUAL, tolerance=1e-9, max_sweeps=200_000,
+                  precision=PREC_FP32_REFINE)
+        s1 = gs[0].relax_ex(tile_shape=1, **kw)
+        s2 = gs[1].relax_ex(tile_shape=two, **kw)
+        assert s2["sweeps_per_pass"] == 2
+        assert s1["sweeps"] == s2["sweeps"], (s1, s2)
+        assert bits_equal(gs[0].download(), gs[1].download())
+
+
+def test_fp32_volume_default_is_the_two_sweep_pipe():
+    """FP32 volumes of 2^18 voxels or more take the pipelined two-sweep
+    smoother with automatic chunks by default (smaller ones keep the
+    one-sweep rule); the two-sweep default gives the one-sweep smoother's
+    bits."""
+    say("case default FP32 volume shape")
+    for dims, want in (((72, 70, 66), 18), ((40, 36, 30), None)):
+        w, h, d = dims
+        faces = np.zeros((d, h, w), np.uint8)
+        faces[:, 0, :] = faces[:, -1, :] = faces[:, :, 0] = faces[:, :, -1] = 1
+        faces[0] = faces[-1] = 1
+        up = np.where(faces.reshape(-1) == 1, 0.0, 1e-3)
+        gs = []
+        for _ in range(2):
+            ref = OracleField(w, h, d, 0.25, 0.0, 1.0)
+            ref.uptake[:] = up
+            ref.hold_where(1.0, faces.reshape(-1))
+            ref.copy_next_from_values()
+            gs.append(gpu_from_oracle(ref))
+        kw = dict(omega=1.9, criterion=CRIT_REL_RESIDUAL, tolerance=1e-9, max_sweeps=200_000,
+                  precision=PREC_FP32_REFINE)
+        s0 = gs[0].relax_ex(**kw)
+        s1 = gs[1].relax_ex(tile_shape=1, **kw)
+        assert s0["rel_residual"] <= 1e-9 and s1["rel_residual"] <= 1e-9
+        if want is None:
+            # the one-sweep rule (the in-place half sweeps round differently)
+            assert s0["tile_shape"] != 18 and s0["sweeps_per_pass"] == 1
+            continue
+        assert s0["tile_shape"] == want and s0["sweeps_per_pass"] == 2
+        assert s0["sweeps"] == s1["sweeps"]
+        assert bits_equal(gs[0].download(), gs[1].download())
