@@ -185,6 +185,8 @@ __global__ __launch_bounds__(kCampThreads) void camp_lists(const CampGroupDev* g
         off += cnt;
     }
     if (threadIdx.x == blockDim.x - 1) *ca.count = s_scan[threadIdx.x];
+    // the FP32 step's per-XCD tile counters start the step at zero
+    if (threadIdx.x < 8) ca.count[kCampCounterStride * (threadIdx.x + 1)] = 0;
 }
 
 // The interior of sweep tile t (the cells it stores), clipped to the span.
@@ -274,23 +276,48 @@ __device__ void camp_correct_tile(const Prob& pr, int t, int par) {
     }
 }
 
-// FP32 refinement step: a fixed grid loops over the step's tile list, each
-// XCD taking a contiguous run of it (neighbouring tiles share one L2); each
-// tile does its field's phase: smoothing sweeps, correction or residual.
+// FP32 refinement step: a fixed grid loops over the step's tile list; each
+// XCD owns a contiguous run of it (neighbouring tiles share one L2), its
+// workgroups take the run's tiles one at a time from the XCD's counter, and
+// once the run is taken they help the other XCDs with theirs. Tiles cost
+// what their field's phase costs (s smoothing sweeps, or one residual or
+// correction pass), and fields in different phases sit side by side in the
+// list, so tiles are handed out as workgroups free up rather than dealt out
+// in advance. Each tile does its field's phase.
 template <int S, int NW, int NR, bool GEN>
 __global__ __launch_bounds__(NW * 64) void camp_step(CampArgs ca) {
     PassArgs a = resolve(ca.pa);
     const int L = a.pass;
     const int n = *ca.count;
-    const int x = blockIdx.x & 7, k = blockIdx.x >> 3, per_xcd = gridDim.x >> 3;
+    const int x = blockIdx.x & 7;
     const int chunk = (n + 7) >> 3;
-    const int lo = x * chunk, hi = min(n, lo + chunk);
+    int* ctr = ca.count + kCampCounterStride;
+    __shared__ int s_tile;
+    int tries = 0;  // thread 0: XCD runs found empty so far (own run first)
     a.mode = MODE_SWEEP;
     a.tl = 0;
     a.check = 0;
-    for (int j = lo + k; j < hi; j += per_xcd) {
-        const int t = ca.list[j];
-        __syncthreads();  // the previous tile's reads of LDS are done
+    while (true) {
+        __syncthreads();  // the previous tile's reads of LDS (and of s_tile) are done
+        if (threadIdx.x == 0) {
+            int t = -1;
+            while (tries < 8) {
+                const int y = (x + tries) & 7;
+                const int lo = y * chunk, hi = min(n, lo + chunk);
+                if (lo < hi) {
+                    const int j = atomicAdd(&ctr[kCampCounterStride * y], 1);
+                    if (lo + j < hi) {
+                        t = ca.list[lo + j];
+                        break;
+                    }
+                }
+                tries++;
+            }
+            s_tile = t;
+        }
+        __syncthreads();
+        const int t = s_tile;
+        if (t < 0) break;
         const int p = a.tile_prob[t];
         const CampField& f = ca.st[p];
         const int phase = f.phase;

@@ -49,13 +49,17 @@ struct CampField {
     double model_rate;  // log of the model problem's SOR rate per sweep (< 0)
 };
 
+// The FP32 step's eight per-XCD tile counters sit after a group's tile count,
+// at count[kCampCounterStride * (x + 1)] (a cache line apart).
+constexpr int kCampCounterStride = 32;
+
 struct CampArgs {
     PassArgs pa;           // sweep view: problems, tile table (main tiling), partials,
                            // control blocks (FP64: the verdict on the input), s, run base
     const Prob* probs64;   // FP32 refinement: the FP64 view of every problem
     CampField* st;
     int* list;             // the tiles that work in the current step
-    int* count;
+    int* count;            // the list's length, then the FP32 step's per-XCD counters
     int fp64;
     int check_every;
     int lo;                // shortest segment

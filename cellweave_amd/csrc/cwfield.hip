@@ -1819,7 +1819,7 @@ cw_status campaign_impl(cw_field* const* F, uint32_t nf, const uint32_t* group,
         const size_t ntiles = pl.tp[TL_MAIN].size();
         auto up = [](size_t b) { return (b + 255) / 256 * 256; };
         const size_t b_st = up(np * sizeof(CampField)), b_p64 = up(np * sizeof(Prob)),
-                     b_run = up(4 * sizeof(int)), b_list = up(std::max<size_t>(ntiles, 1) * sizeof(int));
+                     b_run = up(kCampCounterStride * 9 * sizeof(int)), b_list = up(std::max<size_t>(ntiles, 1) * sizeof(int));
         CW_HIP(hipMalloc((void**)&G->dev, b_st + b_p64 + b_run + b_list));
         G->dst = (CampField*)G->dev;
         G->dp64 = (Prob*)(G->dev + b_st);
