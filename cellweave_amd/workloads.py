@@ -64,6 +64,8 @@ def _solve_pair(fields, fcs, omega, reset, tol=1e-8, reps=3, use_graph=0):
                      "sweeps": [x["sweeps"] for x in st],
                      "rel_residual": [x["rel_residual"] for x in st],
                      "outer": [x["outer_iters"] for x in st],
+                     "sweeps_per_pass": [x["sweeps_per_pass"] for x in st],
+                     "tile_shape": [x["tile_shape"] for x in st],
                      "values": [g.download() for g in fields]}
     return res
 
@@ -115,7 +117,9 @@ def spheroid256(n=256):
     g.fill(1.0)
     ms64 = time_passes([g], 40, criterion=CRIT_REL_RESIDUAL, tolerance=1e-8, omega=omega,
                        precision=PREC_FP64)
-    b32 = span * 16.0  # e 4+4, rhs 4, lower bound 4 (oxygen has sinks), one sweep per launch
+    # B_pass: e 4+4, rhs 4, lower bound 4 (oxygen has sinks) a voxel, read and
+    # written once a launch however many sweeps the launch runs
+    b32 = span * 16.0
     b64 = span * 24.0
     s = _strip(res)
     return _line(res["fp32_refine"]["voxel_sweeps_per_s"],
@@ -125,6 +129,7 @@ def spheroid256(n=256):
                   "errors_vs_fp64": _errors(res, [fc]),
                   "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                                "fp32_kernel_launch_ms": ms32,
+                               "fp32_sweeps_per_launch": res["fp32_refine"]["sweeps_per_pass"][0],
                                "fp32_achieved": b32 / ms32 / 1e6,
                                "fp32_frac": b32 / ms32 / 1e6 / HBM_PEAK_GBS,
                                "fp64_kernel_launch_ms": ms64,
