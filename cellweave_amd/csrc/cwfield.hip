@@ -1322,7 +1322,11 @@ cw_status solve_refine(cw_field* const* F, uint32_t nf, const Opts& o, cw_relax_
     if (three_d && o.shape >= kNum3DShapes)
         return fail(CW_ERR_ARG, "unknown volume tile shape %d", o.shape);
     if (three_d) shape = 0;
-    const int s = kSmoothShapes[shape].s;
+    // Segment lengths are multiples of s. Volumes round them to two sweeps
+    // whatever the shape, so the one- and two-sweep smoothers run the same
+    // segments (and give the same bits), and a batch's two-sweep launches
+    // end every field's segment on a launch boundary.
+    const int s = three_d ? 2 : kSmoothShapes[shape].s;
     Plan local_outer, local_inner;
     Plan& outer = nf == 1 ? field_plan(F[0], 0) : local_outer;
     Plan& inner = nf == 1 ? field_plan(F[0], 1) : local_inner;

@@ -329,7 +329,8 @@ def test_fp32_refine_3d(shape):
 def test_fp32_refine_3d_in_place_odd_segments(inner_tol):
     """The in-place volume smoother keeps its answer in one buffer, so the
     correction must read it there whatever the parity of a segment's last
-    pass (budgets of odd length)."""
+    pass; several segments of different lengths reach the oracle's answer
+    (odd lengths: test_fp32_refine_3d_in_place_odd_cut)."""
     say(f"case 3d refine in place, inner_tol={inner_tol}")
     ref, g = ball_case(side=29, r=11.0)
     ref.relax_residual(1e-13, 1_000_000, omega=1.75)
@@ -338,6 +339,29 @@ def test_fp32_refine_3d_in_place_odd_segments(inner_tol):
     say(f"  refine stats {st}")
     assert st["outer_iters"] >= 4
     assert np.max(np.abs(g.download() - ref.values)) < 1e-8
+
+
+@pytest.mark.parametrize("max_sweeps", [33, 57])
+def test_fp32_refine_3d_in_place_odd_cut(max_sweeps):
+    """Volume segments are multiples of two sweeps, but max_sweeps can cut
+    the last one to an odd length: the in-place smoother's correction still
+    reads its answer from its one buffer. The ping-pong smoother, which
+    stores each sweep in the other buffer, gives the same values to FP32
+    rounding; a correction read from the wrong buffer would be off by a
+    whole sweep's change."""
+    say(f"case 3d refine in place, odd cut at {max_sweeps}")
+    _, g0 = ball_case(side=29, r=11.0)
+    _, g1 = ball_case(side=29, r=11.0)
+    kw = dict(check=False, omega=1.75, criterion=CRIT_REL_RESIDUAL, tolerance=1e-14,
+              max_sweeps=max_sweeps, precision=PREC_FP32_REFINE)
+    s0, _ = batch_relax([g0], tile_shape=0, **kw)
+    s1, _ = batch_relax([g1], tile_shape=1, **kw)
+    assert s0[0]["sweeps"] == s1[0]["sweeps"] == max_sweeps
+    assert s0[0]["status"] != 0 and s1[0]["status"] != 0
+    v0, v1 = g0.download(), g1.download()
+    moved = np.max(np.abs(v1 - 1.0))
+    assert moved > 1e-3, moved
+    assert np.max(np.abs(v0 - v1)) < 1e-6
 
 
 @pytest.mark.parametrize("dims", [(90, 70, 1), (30, 26, 22)])
@@ -465,3 +489,38 @@ def test_fp32_volume_default_is_the_two_sweep_pipe():
         assert s0["tile_shape"] == want and s0["sweeps_per_pass"] == 2
         assert s0["sweeps"] == s1["sweeps"]
         assert bits_equal(gs[0].download(), gs[1].download())
+
+
+def test_fp32_volume_batch_gets_the_bits_alone():
+    """Five volumes of different sizes in one batch (one launch sequence of
+    two-sweep launches): each field gets the sweeps and bits it gets alone
+    at the same shape, segments ending on launch boundaries; under a
+    max_sweeps that is not a multiple of two, no field runs past it."""
+    say("case fp32 volume batch of five")
+    dims = [(40, 36, 30), (52, 44, 38), (66, 60, 58), (72, 70, 66), (48, 80, 70)]
+
+    def make(w, h, d):
+        faces = np.zeros((d, h, w), np.uint8)
+        faces[:, 0, :] = faces[:, -1, :] = faces[:, :, 0] = faces[:, :, -1] = 1
+        faces[0] = faces[-1] = 1
+        rng = np.random.default_rng(w * h + d)
+        ref = OracleField(w, h, d, 0.25, 0.0, 1.0)
+        ref.uptake[:] = np.where(faces.reshape(-1) == 1, 0.0, rng.uniform(0, 2e-3, w * h * d))
+        ref.hold_where(1.0, faces.reshape(-1))
+        ref.copy_next_from_values()
+        return gpu_from_oracle(ref)
+
+    kw = dict(omega=-1.0, criterion=CRIT_REL_RESIDUAL, tolerance=1e-9, max_sweeps=200_000,
+              precision=PREC_FP32_REFINE, tile_shape=18)
+    batch = [make(*x) for x in dims]
+    st, _ = batch_relax(batch, **kw)
+    for x, s, g in zip(dims, st, batch):
+        (one,) = [make(*x)]
+        s1, _ = batch_relax([one], **kw)
+        assert s["sweeps_per_pass"] == 2 and s["rel_residual"] <= 1e-9
+        assert s["sweeps"] == s1[0]["sweeps"], (x, s, s1[0])
+        assert bits_equal(g.download(), one.download()), x
+    cap = [make(*x) for x in dims]
+    st, _ = batch_relax(cap, check=False, **dict(kw, tolerance=1e-30, max_sweeps=101))
+    for s in st:
+        assert s["status"] != 0 and 100 <= s["sweeps"] <= 101, s
