@@ -696,15 +696,25 @@ cw_status stage_tables(cw_lattice* l, hipStream_t s, const uint8_t* flags, const
     return CW_OK;
 }
 
-// Two u8 scratch arrays of the field's pitched size (open mask, reach).
+// The field's two u8 scratch arrays of its pitched size (open mask, reach),
+// allocated on first use and kept: a bath hold every lattice step then
+// allocates and frees nothing (hipFree waits for the device).
 struct Scratch8 {
     uint8_t* a = nullptr;
     uint8_t* b = nullptr;
-    ~Scratch8() {
-        (void)hipFree(a);
-        (void)hipFree(b);
-    }
 };
+
+cw_status scratch8(cw_field* f, Scratch8* s) {
+    for (int i = 0; i < 2; i++)
+        if (!f->mask8[i]) CW_HIP(hipMalloc((void**)&f->mask8[i], f->npad));
+    s->a = f->mask8[0];
+    s->b = f->mask8[1];
+    return CW_OK;
+}
+
+__global__ void k_zero_ints(int* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
 
 cw_status finish(cw_field* f) {
     CW_HIP(hipGetLastError());
@@ -728,8 +738,8 @@ cw_status flood_hold(cw_field* f, const uint8_t* open, uint8_t* reach, double va
                        red_slots(f));
     hipLaunchKernelGGL(k_box_fold, dim3(1), dim3(kBlock), 0, f->stream, (const int*)red_slots(f),
                        (int)red_grid(f), f->scan, f->scan + 1);
-    int box[8];
-    CW_HIP(hipMemcpyAsync(box, f->scan, sizeof box, hipMemcpyDeviceToHost, f->stream));
+    int* box = f->host_small;  // 8 ints, pinned
+    CW_HIP(hipMemcpyAsync(box, f->scan, 8 * sizeof(int), hipMemcpyDeviceToHost, f->stream));
     CW_HIP(hipStreamSynchronize(f->stream));
     Box b{0, -1, 0, -1, 0, -1};
     const int has_box = box[0];
@@ -747,23 +757,35 @@ cw_status flood_hold(cw_field* f, const uint8_t* open, uint8_t* reach, double va
         const int TX = vol ? 32 : 64, TY = vol ? 8 : 16, TZ = vol ? 4 : 1;
         const long ntiles = (long)((b.x1 - b.x0 + TX) / TX) * ((b.y1 - b.y0 + TY) / TY) *
                             ((b.z1 - b.z0 + TZ) / TZ);
+        // Launches go in batches of kFloodBatch, each with its own change
+        // flag, and the host reads the flags once a batch: the reach set
+        // only grows towards its one fixed point, so once a launch changes
+        // nothing the later ones change nothing either, and the batch's
+        // last flag says whether another batch is needed. (A lattice step's
+        // flood settles in about four launches; one host round trip each
+        // cost more than the launches.)
+        constexpr int kFloodBatch = 4;
         int* changed = f->scan + 16;
         const long cap = 4 * ntiles + 64;
-        long it = 0;
-        for (;; it++) {
+        for (long it = 0;; it += kFloodBatch) {
             if (it >= cap) return fail(CW_ERR_DEVICE, "bath flood did not settle");
-            hipLaunchKernelGGL(k_box_init, dim3(1), dim3(64), 0, f->stream, changed);
-            if (vol)
-                hipLaunchKernelGGL((k_flood<32, 8, 4>), dim3((unsigned)ntiles), dim3(kBlock), 0,
-                                   f->stream, g, b, open, (const uint8_t*)f->held, reach, changed);
-            else
-                hipLaunchKernelGGL((k_flood<64, 16, 1>), dim3((unsigned)ntiles), dim3(kBlock), 0,
-                                   f->stream, g, b, open, (const uint8_t*)f->held, reach, changed);
+            hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, f->stream, changed, kFloodBatch);
+            for (int j = 0; j < kFloodBatch; j++) {
+                if (vol)
+                    hipLaunchKernelGGL((k_flood<32, 8, 4>), dim3((unsigned)ntiles), dim3(kBlock),
+                                       0, f->stream, g, b, open, (const uint8_t*)f->held, reach,
+                                       changed + j);
+                else
+                    hipLaunchKernelGGL((k_flood<64, 16, 1>), dim3((unsigned)ntiles), dim3(kBlock),
+                                       0, f->stream, g, b, open, (const uint8_t*)f->held, reach,
+                                       changed + j);
+            }
             CW_HIP(hipGetLastError());
-            int ch = 0;
-            CW_HIP(hipMemcpyAsync(&ch, changed, sizeof ch, hipMemcpyDeviceToHost, f->stream));
+            int* ch = f->host_small + 8;  // pinned, after the box
+            CW_HIP(hipMemcpyAsync(ch, changed, kFloodBatch * sizeof(int), hipMemcpyDeviceToHost,
+                                  f->stream));
             CW_HIP(hipStreamSynchronize(f->stream));
-            if (!ch) break;
+            if (!ch[kFloodBatch - 1]) break;
         }
     }
     hipLaunchKernelGGL(k_flood_finish, dim3(gb), dim3(kBlock), 0, f->stream, g, b, has_box,
@@ -843,8 +865,8 @@ cw_status build_coefficients(cw_field* f, bool may_alloc) {
                        (const double*)f->up, (const double*)f->pr, f->source, red_slots(f));
     hipLaunchKernelGGL(k_box_fold, dim3(1), dim3(kBlock), 0, f->stream, (const int*)red_slots(f),
                        (int)red_grid(f), f->scan + 10, f->scan + 4);
-    int raw[11];
-    CW_HIP(hipMemcpyAsync(raw, f->scan, sizeof raw, hipMemcpyDeviceToHost, f->stream));
+    int* raw = f->host_small;  // 11 ints, pinned
+    CW_HIP(hipMemcpyAsync(raw, f->scan, 11 * sizeof(int), hipMemcpyDeviceToHost, f->stream));
     CW_HIP(hipStreamSynchronize(f->stream));
     // scan[10] holds the flag bits: 1 general, 2 folded sink, 4 general sink, 8 any free
     int sc[10];
@@ -868,7 +890,7 @@ cw_status build_coefficients(cw_field* f, bool may_alloc) {
     // span may have left there.
     for (int b = 0; b < 2; b++)
         if (f->e[b]) CW_HIP(hipMemsetAsync(f->e[b], 0, f->npad * sizeof(float), f->stream));
-    CW_HIP(hipStreamSynchronize(f->stream));
+    // no wait here: whatever reads the coefficients runs on the field's stream
     f->general = general;
     f->has_sink = general ? sc[2] != 0 : sc[1] != 0;
     f->has_span = sc[3] != 0;
@@ -1040,9 +1062,9 @@ cw_status cw_field_hold_obstacles_at(cw_field* f, cw_lattice* l, double value) {
 cw_status cw_field_hold_where(cw_field* f, const uint8_t* mask, double value) {
     if (!f || !mask) return fail(CW_ERR_ARG, "null argument");
     Scratch8 s;
-    CW_HIP(hipMalloc((void**)&s.a, f->npad));
-    cw_status st = upload_mask(f, mask, s.a);
+    cw_status st = scratch8(f, &s);
     if (st) return st;
+    if ((st = upload_mask(f, mask, s.a))) return st;
     return hold(f, 0, s.a, nullptr, value);
 }
 
@@ -1054,10 +1076,9 @@ cw_status cw_field_release_all(cw_field* f) {
 cw_status cw_field_hold_bath_at(cw_field* f, const uint8_t* open, double value) {
     if (!f || !open) return fail(CW_ERR_ARG, "null argument");
     Scratch8 s;
-    CW_HIP(hipMalloc((void**)&s.a, f->npad));
-    CW_HIP(hipMalloc((void**)&s.b, f->npad));
-    cw_status st = upload_mask(f, open, s.a);
+    cw_status st = scratch8(f, &s);
     if (st) return st;
+    if ((st = upload_mask(f, open, s.a))) return st;
     return flood_hold(f, s.a, s.b, value);
 }
 
@@ -1065,8 +1086,7 @@ cw_status cw_field_hold_medium_bath(cw_field* f, cw_lattice* l, double value) {
     cw_status st = check_pair(f, l);
     if (st) return st;
     Scratch8 s;
-    CW_HIP(hipMalloc((void**)&s.a, f->npad));
-    CW_HIP(hipMalloc((void**)&s.b, f->npad));
+    if ((st = scratch8(f, &s))) return st;
     hipLaunchKernelGGL(k_open_from_ids, dim3(grid_of(f->npad)), dim3(kBlock), 0, f->stream,
                        geo_of(f), (const uint32_t*)l->ids, s.a);
     CW_HIP(hipGetLastError());

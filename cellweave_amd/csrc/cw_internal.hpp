@@ -34,6 +34,7 @@ constexpr uint32_t kObstacle = 0xFFFFFFFFu;  // cpm.rs:137 OBSTACLE = u32::MAX
 // Grid-stride reductions run at most kRedBlocks workgroups; each writes its
 // partial result to its own slot and one workgroup folds the slots.
 constexpr int kScanInts = 64;
+constexpr int kHostSmallInts = 64;  // cw_field::host_small
 constexpr int kRedBlocks = 2048;
 constexpr int kRedSlot = 8;
 
@@ -63,6 +64,10 @@ struct cw_field {
                                // ints: per-workgroup partials of the reductions
     mutable void* stage = nullptr;  // flat device staging of host copies
     mutable size_t stage_bytes = 0;
+    int* host_small = nullptr;  // pinned host ints for small read-backs (spans, boxes, flags)
+    uint8_t* mask8[2] = {nullptr, nullptr};  // u8 scratch of npad (the holds' masks,
+                                             // the flood's open set and reach), kept
+                                             // so a hold allocates once per field
     bool has_span = false;
     int span[6] = {0, 0, 0, 0, 0, 0};
     bool has_sink = false;

@@ -391,6 +391,28 @@ struct Plan {
     // allocated; plan_upload fails if the layout does not fit.
     unsigned char* arena = nullptr;
     size_t arena_bytes = 0;
+    // Pinned host staging for the plan's uploads and control-block traffic:
+    // copies from pageable memory are staged by the runtime, one blocking
+    // bounce (and a blit kernel) each, which cost a warm-started relax more
+    // than its sweeps. Uploads are packed into `stage` and copied from there;
+    // the control blocks go both ways through `ctl_pin`. An event after each
+    // batch of copies guards the buffer until the copies have read it. The
+    // allocation-free mode (arena) allocates nothing and copies as before.
+    unsigned char* stage = nullptr;
+    size_t stage_cap = 0, stage_off = 0;
+    hipEvent_t stage_ev = nullptr;
+    bool stage_pending = false;
+    Ctl* ctl_pin = nullptr;
+    int* run_pin = nullptr;  // two ints after the control blocks
+    size_t ctl_cap = 0;
+    int run_dev[2] = {0, 0};  // the graph's pass base as the device holds it
+    bool run_dev_ok = false;
+    hipEvent_t ctl_ev = nullptr;
+    bool ctl_pending = false;
+    // what the device's tile tables and problems hold (not in arena mode):
+    // an upload of the same content to the same buffer is skipped
+    std::vector<int> dev_tp[3];
+    const int* dev_tp_at[3] = {nullptr, nullptr, nullptr};
 
     // Back to a fresh plan, keeping the device buffers and their capacities
     // (and dropping a captured graph, whose launch grids and arguments belong
@@ -410,11 +432,18 @@ struct Plan {
         run_host[0] = run_host[1] = 0;
         compact = false; act.clear(); nact = 0;
         pipe_k = 1; pipe_fits = true;
+        run_dev_ok = false;
     }
 
     ~Plan() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (arena) return;  // nothing here was allocated by the plan
+        if (stage_ev) (void)hipEventSynchronize(stage_ev);
+        if (ctl_ev) (void)hipEventSynchronize(ctl_ev);
+        if (stage) (void)hipHostFree(stage);
+        if (ctl_pin) (void)hipHostFree(ctl_pin);
+        if (stage_ev) (void)hipEventDestroy(stage_ev);
+        if (ctl_ev) (void)hipEventDestroy(ctl_ev);
+        if (arena) return;  // nothing else here was allocated by the plan
         (void)hipFree(run);
         (void)hipFree(dp); (void)hipFree(dc); (void)hipFree(partials); (void)hipFree(sums);
         for (int i = 0; i < 3; i++) (void)hipFree(dtp[i]);
@@ -480,6 +509,70 @@ int pipe_chunks(const Plan& pl, const std::vector<char>* active) {
         }
     }
     return best;
+}
+
+// Makes the plan's pinned upload buffer hold `bytes` once the copies that
+// still read it are done (no staging in arena mode).
+cw_status stage_begin(Plan& pl, size_t bytes) {
+    pl.stage_off = 0;
+    if (pl.arena) return CW_OK;
+    if (pl.stage_pending) {
+        CW_HIP(hipEventSynchronize(pl.stage_ev));
+        pl.stage_pending = false;
+    }
+    if (!pl.stage_ev) CW_HIP(hipEventCreateWithFlags(&pl.stage_ev, hipEventDisableTiming));
+    if (bytes > pl.stage_cap) {
+        if (pl.stage) (void)hipHostFree(pl.stage);
+        pl.stage = nullptr;
+        pl.stage_cap = 0;
+        const size_t cap = std::max<size_t>(bytes + bytes / 2, 4096);
+        CW_HIP(hipHostMalloc((void**)&pl.stage, cap, hipHostMallocDefault));
+        pl.stage_cap = cap;
+    }
+    return CW_OK;
+}
+
+// Host-to-device copy of `bytes` from src through the upload buffer.
+cw_status stage_h2d(Plan& pl, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return CW_OK;
+    if (pl.arena || !pl.stage) {
+        CW_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, pl.stream));
+        return CW_OK;
+    }
+    if (pl.stage_off + bytes > pl.stage_cap) return fail(CW_ERR_DEVICE, "upload staging overrun");
+    unsigned char* at = pl.stage + pl.stage_off;
+    memcpy(at, src, bytes);
+    CW_HIP(hipMemcpyAsync(dst, at, bytes, hipMemcpyHostToDevice, pl.stream));
+    pl.stage_off += (bytes + 255) / 256 * 256;
+    return CW_OK;
+}
+
+cw_status stage_end(Plan& pl) {
+    if (pl.arena || !pl.stage || pl.stage_off == 0) return CW_OK;
+    CW_HIP(hipEventRecord(pl.stage_ev, pl.stream));
+    pl.stage_pending = true;
+    return CW_OK;
+}
+
+// The control blocks' pinned buffer, free of pending copies.
+cw_status ctl_buffer(Plan& pl) {
+    if (pl.arena) return CW_OK;
+    if (pl.ctl_pending) {
+        CW_HIP(hipEventSynchronize(pl.ctl_ev));
+        pl.ctl_pending = false;
+    }
+    if (!pl.ctl_ev) CW_HIP(hipEventCreateWithFlags(&pl.ctl_ev, hipEventDisableTiming));
+    const size_t need = (size_t)std::max(1, pl.nprob);
+    if (need > pl.ctl_cap) {
+        if (pl.ctl_pin) (void)hipHostFree(pl.ctl_pin);
+        pl.ctl_pin = nullptr;
+        pl.run_pin = nullptr;
+        pl.ctl_cap = 0;
+        CW_HIP(hipHostMalloc((void**)&pl.ctl_pin, need * sizeof(Ctl) + 64, hipHostMallocDefault));
+        pl.ctl_cap = need;
+        pl.run_pin = (int*)((unsigned char*)pl.ctl_pin + need * sizeof(Ctl));
+    }
+    return CW_OK;
 }
 
 // Lays out the tilings of every problem (s-sweep tile, one-sweep tile,
@@ -565,21 +658,34 @@ cw_status plan_upload(Plan& pl, const std::vector<char>* active = nullptr) {
     } else {
         if ((st = grow(&pl.dp, &pl.cap_prob, (size_t)std::max(1, np)))) return st;
         if ((st = grow(&pl.dc, &pl.cap_ctl, (size_t)std::max(1, np)))) return st;
-        for (int tl = 0; tl < 3; tl++)
+        for (int tl = 0; tl < 3; tl++) {
+            const size_t cap0 = pl.cap_tp[tl];
             if (!pl.tp[tl].empty() && (st = grow(&pl.dtp[tl], &pl.cap_tp[tl], pl.tp[tl].size())))
                 return st;
+            // a new allocation may reuse the old address: its content is new
+            if (pl.cap_tp[tl] != cap0) pl.dev_tp_at[tl] = nullptr;
+        }
         if ((st = grow(&pl.partials, &pl.cap_part, 2 * (size_t)pl.part_stride))) return st;
         if ((st = grow(&pl.sums, &pl.cap_sums, 2 * (size_t)std::max(1, np)))) return st;
     }
+    size_t bytes = 0;
+    for (int tl = 0; tl < 3; tl++) bytes += (pl.tp[tl].size() * sizeof(int) + 255) / 256 * 256;
+    bytes += (np * sizeof(Prob) + 255) / 256 * 256 + (np * sizeof(Ctl) + 255) / 256 * 256;
+    if ((st = stage_begin(pl, bytes))) return st;
     for (int tl = 0; tl < 3; tl++) {
         if (pl.tp[tl].empty()) continue;
-        CW_HIP(hipMemcpyAsync(pl.dtp[tl], pl.tp[tl].data(), pl.tp[tl].size() * sizeof(int),
-                              hipMemcpyHostToDevice, pl.stream));
+        // a warm-started relax of an unchanged field lays out the same tiles
+        if (!pl.arena && pl.dev_tp_at[tl] == pl.dtp[tl] && pl.dev_tp[tl] == pl.tp[tl]) continue;
+        if ((st = stage_h2d(pl, pl.dtp[tl], pl.tp[tl].data(), pl.tp[tl].size() * sizeof(int))))
+            return st;
+        if (!pl.arena) {
+            pl.dev_tp[tl] = pl.tp[tl];
+            pl.dev_tp_at[tl] = pl.dtp[tl];
+        }
     }
-    CW_HIP(hipMemcpyAsync(pl.dp, pl.hp.data(), np * sizeof(Prob), hipMemcpyHostToDevice,
-                          pl.stream));
-    CW_HIP(hipMemcpyAsync(pl.dc, pl.hc.data(), np * sizeof(Ctl), hipMemcpyHostToDevice, pl.stream));
-    return CW_OK;
+    if ((st = stage_h2d(pl, pl.dp, pl.hp.data(), np * sizeof(Prob)))) return st;
+    if ((st = stage_h2d(pl, pl.dc, pl.hc.data(), np * sizeof(Ctl)))) return st;
+    return stage_end(pl);
 }
 
 PassArgs make_args(Plan& pl, int tl, int mode, int crit, double tol, int max_sweeps) {
@@ -745,15 +851,28 @@ cw_status launch_decide(Plan& pl, int mode, int crit, double tol, int max_sweeps
 }
 
 cw_status read_ctl(Plan& pl) {
-    CW_HIP(hipMemcpyAsync(pl.hc.data(), pl.dc, pl.nprob * sizeof(Ctl), hipMemcpyDeviceToHost,
-                          pl.stream));
+    cw_status st;
+    if ((st = ctl_buffer(pl))) return st;
+    Ctl* to = pl.arena ? pl.hc.data() : pl.ctl_pin;
+    CW_HIP(hipMemcpyAsync(to, pl.dc, pl.nprob * sizeof(Ctl), hipMemcpyDeviceToHost, pl.stream));
     CW_HIP(hipStreamSynchronize(pl.stream));
+    if (to != pl.hc.data()) memcpy(pl.hc.data(), to, pl.nprob * sizeof(Ctl));
     return CW_OK;
 }
 
 cw_status write_ctl(Plan& pl) {
-    CW_HIP(hipMemcpyAsync(pl.dc, pl.hc.data(), pl.nprob * sizeof(Ctl), hipMemcpyHostToDevice,
+    cw_status st;
+    if ((st = ctl_buffer(pl))) return st;
+    if (pl.arena) {
+        CW_HIP(hipMemcpyAsync(pl.dc, pl.hc.data(), pl.nprob * sizeof(Ctl), hipMemcpyHostToDevice,
+                              pl.stream));
+        return CW_OK;
+    }
+    memcpy(pl.ctl_pin, pl.hc.data(), pl.nprob * sizeof(Ctl));
+    CW_HIP(hipMemcpyAsync(pl.dc, pl.ctl_pin, pl.nprob * sizeof(Ctl), hipMemcpyHostToDevice,
                           pl.stream));
+    CW_HIP(hipEventRecord(pl.ctl_ev, pl.stream));
+    pl.ctl_pending = true;
     return CW_OK;
 }
 
@@ -923,11 +1042,25 @@ cw_status run_passes(Plan& pl, int crit, double tol, uint64_t budget, uint64_t d
             pl.pass % check_every == 0 && done0 + (uint64_t)(kGraphPasses * S) <= budget &&
             !(pl.three_d && crit == CRIT_RESID)) {
             if ((st = capture_chunk(pl, kGraphPasses, crit, check_every, tol, maxs))) return st;
-            pl.run_host[0] = pl.pass;
-            pl.run_host[1] = pl.sweeps_before;
-            CW_HIP(hipMemcpyAsync(pl.run, pl.run_host, 2 * sizeof(int), hipMemcpyHostToDevice,
-                                  pl.stream));
+            // the graph reads its pass base from the device; the chunk's
+            // last node advances it, so it is uploaded only when eager
+            // passes (or a new run) moved the host's count away from it
+            if (!pl.run_dev_ok || pl.run_dev[0] != pl.pass || pl.run_dev[1] != pl.sweeps_before) {
+                if ((st = ctl_buffer(pl))) return st;
+                int* from = pl.arena ? pl.run_host : pl.run_pin;
+                from[0] = pl.run_host[0] = pl.pass;
+                from[1] = pl.run_host[1] = pl.sweeps_before;
+                CW_HIP(hipMemcpyAsync(pl.run, from, 2 * sizeof(int), hipMemcpyHostToDevice,
+                                      pl.stream));
+                if (!pl.arena) {
+                    CW_HIP(hipEventRecord(pl.ctl_ev, pl.stream));
+                    pl.ctl_pending = true;
+                }
+            }
             CW_HIP(hipGraphLaunch(pl.gexec, pl.stream));
+            pl.run_dev[0] = pl.pass + kGraphPasses;
+            pl.run_dev[1] = pl.sweeps_before + kGraphPasses * S;
+            pl.run_dev_ok = true;
             pl.pass += kGraphPasses;
             pl.sweeps_before += kGraphPasses * S;
             pl.prev_s = S;
@@ -1598,6 +1731,13 @@ cw_status solve_impl(cw_field* const* F, uint32_t nf, const cw_relax_opts* opts,
         if (!F[i]) return fail(CW_ERR_ARG, "null field %u", i);
         if (F[i]->coef_dirty && (st = build_coefficients(F[i], arena == nullptr))) return st;
     }
+    // A batch swept on the first field's stream reads the others' freshly
+    // built coefficients: their streams finish first. (Alone, or as
+    // concurrent solves on their own streams, each field's work is ordered
+    // on its stream.)
+    if (!run_concurrently(F, nf))
+        for (uint32_t i = 1; i < nf; i++)
+            if (F[i]->stream != F[0]->stream) CW_HIP(hipStreamSynchronize(F[i]->stream));
     std::vector<cw_relax_stats> local;
     if (!stats) {
         local.resize(nf);
@@ -2365,6 +2505,11 @@ cw_status cw_field_create(uint32_t w, uint32_t h, uint32_t d, double diff, doubl
     auto bail = [&](cw_status s) { cw_field_destroy(f); return s; };
     if (hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(CW_ERR_DEVICE, "hipStreamCreate failed"));
+    if (hipHostMalloc((void**)&f->host_small, kHostSmallInts * sizeof(int), hipHostMallocDefault) !=
+        hipSuccess) {
+        f->host_small = nullptr;
+        return bail(fail(CW_ERR_DEVICE, "hipHostMalloc of %d ints failed", kHostSmallInts));
+    }
     const size_t b8 = f->npad * sizeof(double);
     if ((st = alloc_zero((void**)&f->c[0], b8, f->stream, 0))) return bail(st);
     if ((st = alloc_zero((void**)&f->c[1], b8, f->stream, 0))) return bail(st);
@@ -2394,6 +2539,8 @@ cw_status cw_field_destroy(cw_field* f) {
     hipFree(f->c[0]); hipFree(f->c[1]); hipFree(f->k); hipFree(f->k2);
     hipFree(f->up); hipFree(f->pr); hipFree(f->held); hipFree(f->scan); hipFree(f->stage);
     hipFree(f->e[0]); hipFree(f->e[1]); hipFree(f->r32); hipFree(f->lb);
+    hipFree(f->mask8[0]); hipFree(f->mask8[1]);
+    if (f->host_small) hipHostFree(f->host_small);
     if (f->stream) hipStreamDestroy(f->stream);
     delete f;
     return CW_OK;
