@@ -410,7 +410,7 @@ __global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a_in) {
     const int ti = lt % pr.ntx[tl], tj = lt / pr.ntx[tl];
     const int gxo = pr.xs + ti * TX - H;
     const int gyo = pr.ys + tj * TY - H;
-    const int W = pr.W, Hh = pr.H, P = pr.P;
+    const int W = pr.W, Hh = pr.H, P = pr.P, bx1 = pr.x1, by1 = pr.y1;
     const int gx = gxo + 2 * lane;
     const bool colin = gx >= 0 && gx < W;
     const double* kk = (const double*)pr.k;
@@ -538,13 +538,13 @@ __global__ __launch_bounds__(NW * 64) void sweep2d(PassArgs a_in) {
     }
 
     // Store the interior (only rows and pairs that reach the swept box).
-    if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= pr.x1) {
+    if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= bx1) {
 #pragma unroll
         for (int i = 0; i < NR; i++) {
             const int y = wv + NW * i;
             if (y < H || y >= H + TY) continue;
             const int gy = gyo + y;
-            if (gy > pr.y1) continue;
+            if (gy > by1) continue;
             double2 v;
             v.x = cs[y][0][lane];
             v.y = cs[y][1][lane];
@@ -579,7 +579,7 @@ __global__ __launch_bounds__(NW * 64) void smooth2d(PassArgs a_in) {
     const int ti = lt % pr.ntx[tl], tj = lt / pr.ntx[tl];
     const int gxo = pr.xs + ti * TX - H;
     const int gyo = pr.ys + tj * TY - H;
-    const int W = pr.W, Hh = pr.H, P = pr.P;
+    const int W = pr.W, Hh = pr.H, P = pr.P, bx1 = pr.x1, by1 = pr.y1;
     const int gx = gxo + 2 * lane;
     const bool colin = gx >= 0 && gx < W;
     const T* kk = (const T*)pr.k;
@@ -652,13 +652,13 @@ __global__ __launch_bounds__(NW * 64) void smooth2d(PassArgs a_in) {
         __syncthreads();
     }
 
-    if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= pr.x1) {
+    if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= bx1) {
 #pragma unroll
         for (int i = 0; i < NR; i++) {
             const int y = wv + NW * i;
             if (y < H || y >= H + TY) continue;
             const int gy = gyo + y;
-            if (gy > pr.y1) continue;
+            if (gy > by1) continue;
             T2 v;
             v.x = cs[y][0][lane];
             v.y = cs[y][1][lane];
@@ -721,7 +721,7 @@ __global__ __launch_bounds__(NW * 64) void smooth2d_reg(PassArgs a_in) {
     const int ti = lt % pr.ntx[tl], tj = lt / pr.ntx[tl];
     const int gxo = pr.xs + ti * TX - H;
     const int gyo = pr.ys + tj * TY - H;
-    const int W = pr.W, Hh = pr.H, P = pr.P;
+    const int W = pr.W, Hh = pr.H, P = pr.P, bx1 = pr.x1, by1 = pr.y1;
     const int gx = gxo + 2 * lane;
     const bool colin = gx >= 0 && gx < W;
     const T* kk = (const T*)pr.k;
@@ -823,13 +823,13 @@ __global__ __launch_bounds__(NW * 64) void smooth2d_reg(PassArgs a_in) {
         }
     }
 
-    if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= pr.x1) {
+    if (lane >= H / 2 && lane < (H + TX) / 2 && gx <= bx1) {
 #pragma unroll
         for (int i = 0; i < NR; i++) {
             const int r = y0 + i;
             if (r < H || r >= H + TY) continue;
             const int gy = gyo + r;
-            if (gy > pr.y1) continue;
+            if (gy > by1) continue;
             T2 v;
             v.x = ve[i];
             v.y = vo[i];
@@ -1078,6 +1078,7 @@ __global__ __launch_bounds__(NW * 64) void relax3d_stream(PassArgs a_in) {
     const int za = pr.zs + ch * pr.zc;
     const int zb = za + pr.zc - 1 < pr.z1 ? za + pr.zc - 1 : pr.z1;
     const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
+    const int bx1 = pr.x1, by1 = pr.y1, bz1 = pr.z1;
     const size_t plane = (size_t)P * Hh;
     const int gx = gxo + 2 * lane;
     const bool colin = gx >= 0 && gx < W;
@@ -1179,11 +1180,11 @@ __global__ __launch_bounds__(NW * 64) void relax3d_stream(PassArgs a_in) {
     for (int z = za; z <= zb + 1; z++) {
         load_state(p1, z + 1);
         load_coef(kp1, z + 1);
-        if (z <= pr.z1) red_phase(c0, m1, p1, k0, z, track && z <= zb);
+        if (z <= bz1) red_phase(c0, m1, p1, k0, z, track && z <= zb);
         if (z - 1 >= za) {
             black_phase(m1, m2, c0, km1, z - 1, track);
             // store plane z-1: interior rows and pairs inside the swept box
-            if (lane >= 1 && lane < 63 && gx <= pr.x1) {
+            if (lane >= 1 && lane < 63 && gx <= bx1) {
                 const int zz = z - 1;
                 const int sw0 = (base0 + zz) & 1;
 #pragma unroll
@@ -1191,7 +1192,7 @@ __global__ __launch_bounds__(NW * 64) void relax3d_stream(PassArgs a_in) {
                     const int r = r0 + i;
                     if (r < 2 || r >= 2 + TY) continue;
                     const int gy = gyo + r;
-                    if (gy > pr.y1) continue;
+                    if (gy > by1) continue;
                     const bool sw = (sw0 ^ (i & 1)) != 0;
                     T2 v;
                     v.x = sw ? m1.st[1][i] : m1.st[0][i];
@@ -1618,20 +1619,32 @@ static __global__ __launch_bounds__(256) void resid64(PassArgs a, int write32) {
     if (a.ctl[p].status != ST_RUNNING) return;
     const ElTile et = el_tile(pr, t);
     if (et.ok) {
+        // Everything the stores need is read first: a store may alias the
+        // problem record as far as the compiler knows, so a field read after
+        // one would be reloaded behind it.
         const int W = pr.W, Hh = pr.H, D = pr.D, P = pr.P;
         const size_t plane = (size_t)P * Hh;
         const double* c = pr.c64;
         const double* kk = (const double*)pr.k;
         const double* kk2 = (const double*)pr.k2;
-        const double d = pr.d, decay = pr.decay;
+        float* r32 = pr.r32;
+        float* e0 = pr.e0;
+        float* lb32 = pr.lb32;
+        const double d = pr.d, decay = pr.decay, omega = pr.omega, source = pr.source;
+        const bool project = a.project;
+        size_t at[2] = {0, 0};
+        bool live[2] = {false, false}, held[2] = {false, false};
+        float rs[2] = {0.0f, 0.0f}, ls[2] = {0.0f, 0.0f};
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const int x = et.gx + q;
             if (x >= W) continue;
             const size_t i = (size_t)et.gz * plane + (size_t)et.gy * P + x;
+            at[q] = i;
+            live[q] = true;
             const double kq = kk[i];
             if (kq != kq) {
-                if (write32) { pr.r32[i] = __int_as_float(0x7fc00000); pr.e0[i] = 0.0f; }
+                held[q] = true;
                 continue;
             }
             const double ci = c[i];
@@ -1652,18 +1665,27 @@ static __global__ __launch_bounds__(256) void resid64(PassArgs a, int write32) {
             CW_NB(et.gz + 1 < D, i + plane)
 #undef CW_NB
             const double A = (double)real * d + decay;
-            const double b = kk2 ? ((pr.source + kq) - kk2[i]) : kq;
+            const double b = kk2 ? ((source + kq) - kk2[i]) : kq;
             const double r = (b + d * sum) - A * ci;
             double rt = r;
-            if (a.project && A > 0.0 && !(ci + r / A > 0.0)) rt = -A * ci;
+            if (project && A > 0.0 && !(ci + r / A > 0.0)) rt = -A * ci;
             const double be = b + d * hsum;
             r2 += rt * rt;
             b2 += be * be;
             if (write32) {
                 // the FP32 smoother's pre-scaled right-hand side omega r / A
-                pr.r32[i] = (float)(pr.omega * r / A);
-                pr.e0[i] = 0.0f;
-                if (pr.lb32) pr.lb32[i] = __double2float_rz(-ci);
+                rs[q] = (float)(omega * r / A);
+                ls[q] = __double2float_rz(-ci);
+            }
+        }
+        if (write32) {
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                if (!live[q]) continue;
+                const size_t i = at[q];
+                r32[i] = held[q] ? __int_as_float(0x7fc00000) : rs[q];
+                e0[i] = 0.0f;
+                if (lb32 && !held[q]) lb32[i] = ls[q];
             }
         }
     }
@@ -1683,15 +1705,24 @@ static __global__ __launch_bounds__(256) void correct64(PassArgs a) {
     const Ctl& ic = a.inner_ctl[pr.inner_idx];
     const float* e = (ic.parity & 1) ? pr.e1 : pr.e0;
     const double* kk = (const double*)pr.k;
+    double* c64 = pr.c64;
+    // both cells are read before either is written (see resid64)
+    size_t at[2] = {0, 0};
+    bool live[2] = {false, false};
+    double v[2] = {0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < 2; q++) {
         const int x = et.gx + q;
         if (x >= W) continue;
         const size_t i = (size_t)et.gz * plane + (size_t)et.gy * P + x;
         if (kk[i] != kk[i]) continue;
-        const double v = pr.c64[i] + (double)e[i];
-        pr.c64[i] = v > 0.0 ? v : 0.0;
+        at[q] = i;
+        live[q] = true;
+        v[q] = c64[i] + (double)e[i];
     }
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+        if (live[q]) c64[at[q]] = v[q] > 0.0 ? v[q] : 0.0;
 }
 
 // One workgroup: records verdicts for the host. DECIDE_PASS assesses the last
