@@ -19,7 +19,8 @@ constexpr Shape kSmoothShapes[] = {{1, 32, 4, 0},   {2, 32, 4, 0},   {4, 64, 8, 
                                    {2, 32, 4, 2},   {4, 64, 8, 2},   {4, 64, 16, 2},  {8, 128, 16, 2},
                                    {2, 64, 8, 2},   {4, 128, 16, 2}, {6, 128, 16, 2},
                                    {10, 128, 16, 2}, {12, 128, 16, 2}, {6, 64, 8, 2},
-                                   {8, 64, 8, 2},   {12, 64, 8, 2},  {16, 128, 16, 2}};
+                                   {8, 64, 8, 2},   {12, 64, 8, 2},  {16, 128, 16, 2},
+                                   {8, 160, 16, 2}, {10, 160, 16, 2}, {12, 160, 16, 2}};
 constexpr int kNumPlainShapes = (int)(sizeof(kPlainShapes) / sizeof(kPlainShapes[0]));
 constexpr int kNumSmoothShapes = (int)(sizeof(kSmoothShapes) / sizeof(kSmoothShapes[0]));
 

@@ -201,19 +201,30 @@ int default_smooth_shape(int s) { return s <= 1 ? 0 : s == 2 ? 16 : s <= 4 ? 14 
 //     when they fit on the CUs at once: more, shorter workgroups;
 //   - else 128 x 128 regions at S = 16 (shape 24) when they fit at once: one
 //     round of the deepest tile, whose per-sweep latency is least;
-//   - else the S (8, 10, 12 or 16 on 128 x 128) with the least tile-work a
-//     sweep, tiles(S) x (L + S w) / S with a tile load L = 20 sweeps w: with
-//     many tiles the launches of several rounds and fields overlap, so work,
-//     not rounds, sets the time.
+//   - else, for fields launched one sequence at a time (a single field, or a
+//     batch of more than four), the shape with the least time a sweep when
+//     each launch takes whole rounds of the CUs: rounds x (RY / 128) x
+//     (L + S w) / S, with a tile load L of 20 sweeps w, over the 128-row
+//     shapes at S = 8, 10, 12, 16 and the 160-row ones at S = 8, 10, 12;
+//   - else (two to four fields solved concurrently, their launches
+//     overlapping) the least tile-work a sweep, tiles x (L + S w) / S: over
+//     the 128-row shapes while the batch has fewer than three rounds of
+//     128 x 128 tiles at S = 8 (a launch of 160-row regions that spills just
+//     past one round holds its stream back a whole workgroup), with the
+//     160-row ones too, weighted by RY / 128, beyond that.
 // FP32+refine to 1e-8, ms (shapes 23 / 24 / 15 / 19 / 20, the rule's choice
 // in brackets):
 //   128^2 0.74 / 0.78 / 0.96 / 0.94 / 0.87 (23)   256^2 1.10 / 1.37 / 1.82 / 1.59 / 1.39 (23)
 //   512^2 2.14 / 2.52 / 3.33 / 3.00 / 2.77 (23)   768^2 3.36 / 3.47 / 4.35 / 3.91 / 3.62 (23)
-//   1024^2 7.04 / 4.29 / 5.87 / 5.07 / 4.73 (24)  2048^2 37.1 / 27.9 / 22.8 / 26.9 / 24.5 (15)
+//   1024^2 7.04 / 4.29 / 5.87 / 5.07 / 4.73 (24)
 //   three 256^2 1.68 / 1.73 / 2.17 / 2.04 / 1.93 (23)   two 512^2 2.60 / 2.81 / 3.70 / 3.28 / 3.06 (23)
-//   three 512^2 3.05 / 2.88 / 3.80 / 3.40 / 3.11 (24)   three 1024^2 15.9 / 11.3 / 8.48 / 8.93 / 9.41 (15)
-//   three 2048^2 117 / 84.8 / 59.8 / 60.5 / 65.7 (15)
+//   three 512^2 3.05 / 2.88 / 3.80 / 3.40 / 3.11 (24)
+// and (shapes 15 / 19 / 20 / 25 / 26 / 27, the 160-row ones at S = 8, 10, 12):
+//   1024^2 5.18 / 4.55 / 4.26 / 5.99 / 5.29 / 4.86 (24: 3.90)   2048^2 21.5 / 24.8 / 22.8 / 23.6 / 21.6 / 19.7 (27)
+//   three 768^2 4.80 / 4.69 / 4.54 / 5.48 / 5.23 / 4.59 (15)   three 1024^2 7.81 / 8.20 / 8.77 / 8.08 / 7.93 / 8.53 (15)
+//   three 1536^2 23.5 / 25.0 / 27.6 / 21.6 / 22.3 / 23.9 (25)  three 2048^2 54.8 / 55.7 / 61.1 / 46.9 / 50.8 / 53.8 (25)
 constexpr int kPlaneDeepShapes[] = {15, 19, 20, 24};  // S = 8, 10, 12, 16 on 128 x 128 regions
+constexpr int kPlaneTallShapes[] = {25, 26, 27};      // S = 8, 10, 12 on 128 x 160 regions
 constexpr int kPlaneShortShape = 23;                  // S = 12 on 128 x 64 regions
 constexpr int kPlaneDeepest = 24;
 constexpr double kTileLoadSweeps = 20.0;
@@ -228,6 +239,7 @@ long plane_tiles(cw_field* const* F, uint32_t nf, const Shape& sh) {
     }
     return tiles;
 }
+bool run_concurrently(cw_field* const* F, uint32_t nf);
 int default_smooth2d_shape(cw_field* const* F, uint32_t nf) {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
@@ -236,16 +248,24 @@ int default_smooth2d_shape(cw_field* const* F, uint32_t nf) {
         cus = 256;
     if (plane_tiles(F, nf, kSmoothShapes[kPlaneShortShape]) <= 2L * cus) return kPlaneShortShape;
     if (plane_tiles(F, nf, kSmoothShapes[kPlaneDeepest]) <= cus) return kPlaneDeepest;
+    const bool overlap = run_concurrently(F, nf);
+    const bool tall = !overlap || plane_tiles(F, nf, kSmoothShapes[kPlaneDeepShapes[0]]) >= 3L * cus;
     int best = kPlaneDeepShapes[0];
-    double best_work = 0.0;
-    for (int si : kPlaneDeepShapes) {
+    double best_cost = 0.0;
+    auto consider = [&](int si) {
         const Shape sh = kSmoothShapes[si];
-        const double work = (double)plane_tiles(F, nf, sh) * (kTileLoadSweeps + sh.s) / sh.s;
-        if (si == kPlaneDeepShapes[0] || work < best_work) {
+        const long tiles = plane_tiles(F, nf, sh);
+        const double units = overlap ? (double)tiles : (double)((tiles + cus - 1) / cus);
+        const double rows = tall ? sh.ry / 128.0 : 1.0;
+        const double cost = units * rows * (kTileLoadSweeps + sh.s) / sh.s;
+        if (si == kPlaneDeepShapes[0] || cost < best_cost) {
             best = si;
-            best_work = work;
+            best_cost = cost;
         }
-    }
+    };
+    for (int si : kPlaneDeepShapes) consider(si);
+    if (tall)
+        for (int si : kPlaneTallShapes) consider(si);
     return best;
 }
 
@@ -785,6 +805,9 @@ cw_status launch_sweep(Plan& pl, const PassArgs& a_in) {
         case 22: CW_LB(8, 8, 8, b512); break;
         case 23: CW_LB(12, 8, 8, b512); break;
         case 24: CW_LB(16, 16, 8, b1024); break;
+        case 25: CW_LB(8, 16, 10, b1024); break;
+        case 26: CW_LB(10, 16, 10, b1024); break;
+        case 27: CW_LB(12, 16, 10, b1024); break;
         default: CW_LB(12, 16, 8, b1024); break;
         }
     }

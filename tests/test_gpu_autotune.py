@@ -12,7 +12,7 @@ from helpers import gpu_from_case
 pytestmark = pytest.mark.gpu
 
 # shape tables of the library: plain FP64 planes, FP32 smoother planes, volumes
-N_PLAIN, N_SMOOTH, N_VOLUME = 10, 25, 19
+N_PLAIN, N_SMOOTH, N_VOLUME = 10, 28, 19
 TWO_SWEEP_VOLUME = {12, 13, 14, 18}  # FP32 smoothing only
 
 

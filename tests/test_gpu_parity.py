@@ -223,7 +223,7 @@ def test_batch_equals_single():
         assert bits_equal(g.download(), v)
 
 
-SMOOTH_SHAPES = range(25)  # FP32 smoother kernels
+SMOOTH_SHAPES = range(28)  # FP32 smoother kernels
 
 
 @pytest.mark.parametrize("shape", SMOOTH_SHAPES)
