@@ -29,7 +29,7 @@ TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_
 CLASSES = ((0, 256, "<256"), (256, 512, "256-511"), (512, 1 << 30, ">=512"))
 
 # Plane tile shapes the device-driven batch is compiled for (cw_campaign_relax).
-DEVICE_SHAPES = {"fp64": (8, 9), "fp32": (15, 18, 19, 20, 21, 22, 23, 24)}
+DEVICE_SHAPES = {"fp64": (8, 9), "fp32": (15, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27)}
 
 
 def size_class(n: int) -> str:

@@ -423,6 +423,9 @@ void* pick(int shape, bool fp64, bool gen, bool tail, int* threads) {
     case 22: CW_K(8, 8, 8);
     case 23: CW_K(12, 8, 8);
     case 24: CW_K(16, 16, 8);
+    case 25: CW_K(8, 16, 10);
+    case 26: CW_K(10, 16, 10);
+    case 27: CW_K(12, 16, 10);
     default: return nullptr;
     }
 #undef CW_K

@@ -176,7 +176,7 @@ cw_status cw_batch_relax(cw_field *const *fields, uint32_t count, const cw_relax
  * relative-residual rule, FP64 or FP32 refinement. Field i belongs to group
  * group[i] < ngroups (a size class), and each group is swept with its own
  * plane tile shape shapes[g] (-1 or shapes NULL: the library's choice; FP64
- * takes 8 or 9, FP32 15 or 18 to 24), as cw_autotune ranks them. Every field
+ * takes 8 or 9, FP32 15 or 18 to 27), as cw_autotune ranks them. Every field
  * keeps its own state on the device: one kernel between two steps moves each
  * field to its next phase (FP64: sweeps with a residual test every
  * check_every steps; FP32: FP64 residual, a smoothing segment whose length it
