@@ -1091,6 +1091,11 @@ cw_status run_passes(Plan& pl, int crit, double tol, uint64_t budget, uint64_t d
             pl.prev_check = (crit == CRIT_RESID && !pl.three_d && ((kGraphPasses - 1) % check_every == 0)) ? 1 : 0;
             launched += kGraphPasses;
             if (out) out->graph_replays++;
+            // Without a stopping rule nothing is decided between replays (a
+            // field whose budget is spent finds so in each launch's
+            // prologue): the next replay is queued behind this one at once,
+            // and the verdicts are read once, after the remainder.
+            if (crit == CRIT_NONE) continue;
             if ((st = read_ctl(pl))) return st;
             if (!any_status(pl, ST_RUNNING)) break;
             if ((st = compact_plan(pl))) return st;
