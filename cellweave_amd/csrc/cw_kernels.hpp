@@ -881,7 +881,11 @@ __device__ __forceinline__ void reg_half(RegCells<T, NR>& R, T above, T below, i
                                          T source, const T (&al)[2][3], bool has_lb, int H,
                                          int TX, int TY, unsigned long long* mv) {
 #pragma unroll
-    for (int i = 0; i < NR; i++) {
+    for (int k = 0; k < NR; k++) {
+        // FP32: the wave's first and last rows, which wait for the edge rows
+        // read from LDS after the barrier, go last, so the interior rows'
+        // updates cover that read (a colour's cells are independent).
+        const int i = !kConstGamma<T, PLAIN> ? k : k < NR - 2 ? k + 1 : k == NR - 2 ? 0 : NR - 1;
         const int q = P0 ^ (i & 1);  // compile-time after unrolling
         const int r = r0 + i;
         if (PLAIN && (r < lo || r > hiY)) continue;
