@@ -125,7 +125,7 @@ def solve(fields, plan_, precision=PREC_FP32_REFINE, tol=1e-8, use_graph=1,
     return stats, time.perf_counter() - t0
 
 
-def tune(cases_, per_class=0, launches=10, confirm=4, tol=1e-8):
+def tune(cases_, per_class=0, launches=10, confirm=6, tol=1e-8):
     """Rank the sweep shapes on each size class, for both precisions, on every
     field of the class (per_class=0), since how many workgroup rounds a launch
     takes depends on the whole batch, or on its first per_class fields.

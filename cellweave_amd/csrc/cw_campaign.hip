@@ -316,7 +316,8 @@ __global__ __launch_bounds__(NW * 64) void camp_step(CampArgs ca) {
             s_tile = t;
         }
         __syncthreads();
-        const int t = s_tile;
+        // uniform, so the tile's problem record is read with scalar loads
+        const int t = __builtin_amdgcn_readfirstlane(s_tile);
         if (t < 0) break;
         const int p = a.tile_prob[t];
         const CampField& f = ca.st[p];

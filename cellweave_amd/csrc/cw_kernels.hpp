@@ -865,6 +865,23 @@ struct RegCells {
                    // FP32 smoother uses -omega everywhere (below)
 };
 
+// A workgroup barrier that orders LDS only: __syncthreads() also waits for
+// every global store of the wave to complete (its release fence), and on this
+// target a wait on stores is a wait on the vector-memory counter, which the
+// loads in flight share. Kernels that prefetch across a barrier use this one.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// A workgroup barrier for LDS that the waves write and read with ds
+// instructions: each wave's ds accesses complete before it arrives, and the
+// compiler moves no memory access across it. Nothing else is waited for.
+__device__ __forceinline__ void ds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // The FP32 smoother keeps gamma = -omega on every cell: a held or outside
 // cell has alpha = beta = 0 and a correction of exactly zero (the correction
 // buffers are zero wherever no free cell's value was stored: zeroed when
@@ -1234,16 +1251,6 @@ __global__ __launch_bounds__(NW * 64) void relax3d_stream(PassArgs a_in) {
 // of its step, reads those of the previous step, and one barrier at the end
 // of the step orders the two (double-buffered by step parity). The next
 // plane's loads are issued a step ahead.
-// A workgroup barrier that orders LDS only: __syncthreads() also waits for
-// every global store of the wave to complete (its release fence), and on this
-// target a wait on stores is a wait on the vector-memory counter, which the
-// loads in flight share. Kernels that prefetch across a barrier use this one.
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // Slot of plane z+2-j in a ring of L planes at a step with z mod L == R.
 template <int L, int R, int J>
 __device__ __forceinline__ constexpr int ring_slot() { return ((R + 2 - J) % L + 2 * L) % L; }
