@@ -130,17 +130,13 @@ __device__ inline void camp_next(const CampArgs& ca, const PassArgs& a, CampFiel
 constexpr int kCampThreads = 1024;
 
 // Between steps L - 1 and L, part 1: every field's phase for step L, one wave
-// per field of fields first_field .. first_field + nfields - 1 (numbered over
-// every group; a field's tile partials summed in the fixed order of
-// problem_sums, so FP64 stops where the host-driven batch stops).
+// per field over every group (a field's tile partials summed in the fixed
+// order of problem_sums, so FP64 stops where the host-driven batch stops).
 __global__ __launch_bounds__(kCampThreads) void camp_verdicts(const CampGroupDev* groups,
-                                                              int ngroups, int first_field,
-                                                              int nfields, int step,
+                                                              int ngroups, int step,
                                                               const int* run) {
     const int lane = threadIdx.x & 63;
-    const int k = blockIdx.x * (kCampThreads / 64) + (threadIdx.x >> 6);
-    if (k >= nfields) return;  // fields first_field .. first_field + nfields - 1 only
-    const int w = first_field + k;
+    const int w = blockIdx.x * (kCampThreads / 64) + (threadIdx.x >> 6);
     int g = 0;
     while (g < ngroups && w >= groups[g].field_base + groups[g].nfields) g++;
     if (g >= ngroups) return;
@@ -465,15 +461,11 @@ cw_status camp_grid(int shape, bool fp64, bool gen, bool tail, long ntiles, int*
 }
 
 cw_status camp_launch_transition(const CampGroupDev* groups, int ngroups, int nfields, int step,
-                                 const int* run, hipStream_t stream, int group, int first_field) {
+                                 const int* run, hipStream_t stream) {
     const int waves = kCampThreads / 64;
-    if (nfields <= 0) return CW_OK;
     hipLaunchKernelGGL(camp_verdicts, dim3((nfields + waves - 1) / waves), dim3(kCampThreads), 0,
-                       stream, groups, ngroups, first_field, nfields, step, run);
-    if (group < 0)
-        hipLaunchKernelGGL(camp_lists, dim3(ngroups), dim3(kCampThreads), 0, stream, groups);
-    else
-        hipLaunchKernelGGL(camp_lists, dim3(1), dim3(kCampThreads), 0, stream, groups + group);
+                       stream, groups, ngroups, step, run);
+    hipLaunchKernelGGL(camp_lists, dim3(ngroups), dim3(kCampThreads), 0, stream, groups);
     CW_HIP(hipGetLastError());
     return CW_OK;
 }

@@ -95,11 +95,9 @@ constexpr int kCampSplit = 4;
 cw_status camp_grid(int shape, bool fp64, bool gen, bool tail, long ntiles, int* grid);
 // Between two steps: every group's fields move to their next phase, then
 // every group's tile list is rebuilt (groups: nfields fields in all, device
-// memory; the step index is `step` plus run[0]). With group >= 0, only that
-// group's: its nfields fields, the first of them field first_field of all.
+// memory; the step index is `step` plus run[0]).
 cw_status camp_launch_transition(const cw::CampGroupDev* groups, int ngroups, int nfields,
-                                 int step, const int* run, hipStream_t stream, int group = -1,
-                                 int first_field = 0);
+                                 int step, const int* run, hipStream_t stream);
 // One launch of a group's step kernel (FP64: full width, or the tail form).
 cw_status camp_launch_step(int shape, bool fp64, bool gen, bool tail, const cw::CampArgs& ca,
                            int grid, hipStream_t stream);

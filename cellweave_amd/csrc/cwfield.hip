@@ -2051,15 +2051,11 @@ cw_status campaign_impl(cw_field* const* F, uint32_t nf, const uint32_t* group,
             (void)hipEventDestroy(setup[g]);
             if (e2 != hipSuccess) return fail(CW_ERR_DEVICE, "set-up join failed");
         }
-        // One graph of kCampSteps rounds. A full-width FP64 step is split over
-        // kCampSplit lanes (one queue cannot dispatch a workgroup per tile fast
-        // enough): each round the transition for every group on the first
-        // group's stream, then the step launches forked onto their lanes and
-        // joined back. The FP64 tail form and FP32 take one lane per group, and
-        // the groups share no state, so each lane runs its own group's rounds
-        // (transition, step) without waiting for the others: one group's
-        // transition runs under the others' steps, and a group whose step is
-        // short does not wait for the longest one.
+        // One graph of kCampSteps rounds: each round the transition for every
+        // group on the first group's stream, then the step launches forked onto
+        // lanes of their own and joined back. A full-width FP64 step is split
+        // over kCampSplit lanes (one queue cannot dispatch a workgroup per tile
+        // fast enough); the FP64 tail form and FP32 take one lane per group.
         auto capture = [&](bool tail, hipGraphExec_t* out) -> cw_status {
             struct Lane {
                 int g, part;
@@ -2096,32 +2092,7 @@ cw_status campaign_impl(cw_field* const* F, uint32_t nf, const uint32_t* group,
                 return fail(CW_ERR_DEVICE, "graph capture failed to begin");
             }
             cw_status cst = CW_OK;
-            const bool chains = lanes.size() == gs.size();
-            if (chains) {
-                // fork every lane once, each runs its group's rounds, join once
-                if (lanes.size() > 1 && hipEventRecord(ev[0], s0) != hipSuccess)
-                    cst = fail(CW_ERR_DEVICE, "fork record failed");
-                for (size_t i = 1; i < lanes.size() && cst == CW_OK; i++)
-                    if (hipStreamWaitEvent(lanes[i].stream, ev[0], 0) != hipSuccess)
-                        cst = fail(CW_ERR_DEVICE, "fork wait failed");
-                for (int j = 0; j < kCampSteps && cst == CW_OK; j++)
-                    for (size_t i = 0; i < lanes.size() && cst == CW_OK; i++) {
-                        const int g = lanes[i].g;
-                        CampGroup& G = *gs[g];
-                        cst = camp_launch_transition(dgroups, (int)gs.size(), hg[g].nfields, j, run,
-                                                     lanes[i].stream, g, hg[g].field_base);
-                        if (cst != CW_OK) break;
-                        CampArgs ca = camp_args(G, j, o, check_every, fp64);
-                        const int grid = tail ? G.grid_tail : G.grid;
-                        ca.slot_base = 0;
-                        cst = camp_launch_step(G.shape, fp64, G.gen, tail, ca, grid, lanes[i].stream);
-                    }
-                for (size_t i = 1; i < lanes.size() && cst == CW_OK; i++)
-                    if (hipEventRecord(ev[i], lanes[i].stream) != hipSuccess ||
-                        hipStreamWaitEvent(s0, ev[i], 0) != hipSuccess)
-                        cst = fail(CW_ERR_DEVICE, "join failed");
-            }
-            for (int j = 0; !chains && j < kCampSteps && cst == CW_OK; j++) {
+            for (int j = 0; j < kCampSteps && cst == CW_OK; j++) {
                 cst = camp_launch_transition(dgroups, (int)gs.size(), nfields_all, j, run, s0);
                 if (cst == CW_OK && lanes.size() > 1 && hipEventRecord(ev[0], s0) != hipSuccess)
                     cst = fail(CW_ERR_DEVICE, "fork record failed");
