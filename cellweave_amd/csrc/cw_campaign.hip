@@ -162,29 +162,62 @@ __global__ __launch_bounds__(kCampThreads) void camp_verdicts(const CampGroupDev
 __global__ __launch_bounds__(kCampThreads) void camp_lists(const CampGroupDev* groups) {
     const CampArgs& ca = groups[blockIdx.x].ca;
     const PassArgs& a = ca.pa;
-    __shared__ int s_scan[kCampThreads];
+    constexpr int NWV = kCampThreads / 64;
+    __shared__ int s_off[kCampThreads], s_cnt[kCampThreads], s_base[kCampThreads];
+    __shared__ int s_wave[NWV];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int np = a.nprob;
     const int per = (np + blockDim.x - 1) / blockDim.x;
     const int p0 = min(np, (int)threadIdx.x * per), p1 = min(np, p0 + per);
-    int mine = 0;
-    for (int p = p0; p < p1; p++)
-        if (camp_active(ca.st[p].phase)) mine += a.probs[p].sw_count[0];
-    s_scan[threadIdx.x] = mine;
-    __syncthreads();
-    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
-        const int v = (int)threadIdx.x >= o ? s_scan[threadIdx.x - o] : 0;
-        __syncthreads();
-        s_scan[threadIdx.x] += v;
-        __syncthreads();
-    }
-    int off = s_scan[threadIdx.x] - mine;
+    // each thread's working tiles (its problems' loads issued together)
+    int mine = 0, base1 = 0;
     for (int p = p0; p < p1; p++) {
-        if (!camp_active(ca.st[p].phase)) continue;
-        const int base = a.probs[p].sw_base[0], cnt = a.probs[p].sw_count[0];
-        for (int j = 0; j < cnt; j++) ca.list[off + j] = base + j;
-        off += cnt;
+        const int ph = ca.st[p].phase;
+        const int cnt = a.probs[p].sw_count[0], base = a.probs[p].sw_base[0];
+        if (camp_active(ph)) mine += cnt;
+        base1 = base;
     }
-    if (threadIdx.x == blockDim.x - 1) *ca.count = s_scan[threadIdx.x];
+    // exclusive offsets: a shuffle scan in each wave, then over the waves
+    int x = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) s_wave[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        int w = lane < NWV ? s_wave[lane] : 0;
+#pragma unroll
+        for (int d = 1; d < NWV; d <<= 1) {
+            const int y = __shfl_up(w, d, 64);
+            if (lane >= d) w += y;
+        }
+        if (lane < NWV) s_wave[lane] = w;
+    }
+    __syncthreads();
+    const int off = (wave > 0 ? s_wave[wave - 1] : 0) + x - mine;
+    if (per == 1) {
+        // one problem a thread: each wave writes whole problems' runs of
+        // tiles, 64 consecutive entries a store
+        s_off[threadIdx.x] = off;
+        s_cnt[threadIdx.x] = mine;
+        s_base[threadIdx.x] = base1;
+        __syncthreads();
+        for (int p = wave; p < np; p += NWV) {
+            const int cnt = s_cnt[p], o = s_off[p], b = s_base[p];
+            for (int j = lane; j < cnt; j += 64) ca.list[o + j] = b + j;
+        }
+    } else {
+        int o = off;
+        for (int p = p0; p < p1; p++) {
+            if (!camp_active(ca.st[p].phase)) continue;
+            const int base = a.probs[p].sw_base[0], cnt = a.probs[p].sw_count[0];
+            for (int j = 0; j < cnt; j++) ca.list[o + j] = base + j;
+            o += cnt;
+        }
+    }
+    if (threadIdx.x == blockDim.x - 1) *ca.count = off + mine;
     // the FP32 step's per-XCD tile counters start the step at zero
     if (threadIdx.x < 8) ca.count[kCampCounterStride * (threadIdx.x + 1)] = 0;
 }
