@@ -436,7 +436,7 @@ def main():
     # The smoother's per-launch working set: every field's pass bytes at once
     # (the three launches run concurrently). Within the 256 MB Infinity Cache
     # the sweeps are bound by VALU work and the tile loads' latency
-    # (SQ_INSTS_VALU at 0.81 of the roof at 1024^2); beyond it each launch
+    # (SQ_INSTS_VALU at 0.84 of the roof at 1024^2); beyond it each launch
     # streams from HBM as well.
     working_set = sum(s_vox * bytes_per_voxel_pass(fc, "fp32") for s_vox, fc in zip(span, fcs))
     binding = ("VALU and tile-load latency (FP32 register-blocked smoother; working set %.0f MB, "
